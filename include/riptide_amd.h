@@ -1,0 +1,130 @@
+/*
+ * riptide_amd C-ABI: the drop-in boundary of the riptide FFA periodogram path
+ * on AMD Instinct MI355X (gfx950).
+ *
+ * Every entry point replaces one function of the reference's pybind11 module
+ * `riptide.libcpp` (src/riptide/cpp/python_bindings.cpp) or one numpy step of
+ * its Python layer, cited next to each declaration.  Conventions:
+ *
+ *   - Every function returns an int status: RT_OK, RT_EINVAL (argument error,
+ *     raised as ValueError with the reference's message, see rt_last_error)
+ *     or RT_EDEVICE (HIP error, raised as RuntimeError).
+ *   - Array arguments named d_* are DEVICE pointers (HBM).  Arguments named
+ *     h_* are host pointers.  Sizes are int64_t element counts.
+ *   - `stream` is a hipStream_t (NULL = the default stream).  Unless stated
+ *     otherwise a call returns once its work is complete on `stream`.
+ *   - Outputs are caller-allocated; the library keeps a per-device scratch
+ *     pool for intermediate buffers, so concurrent calls on one device are
+ *     serialised internally.
+ */
+#ifndef RIPTIDE_AMD_H
+#define RIPTIDE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_OK 0
+#define RT_EINVAL 1
+#define RT_EDEVICE 2
+
+/* Message of the last failed call on this thread. */
+const char* rt_last_error(void);
+
+/* Library ABI version. */
+int rt_version(void);
+
+/* ---- Host-side search plan (no device work) ------------------------------ */
+
+/* periodogram_length (periodogram.hpp:63-109; python_bindings.cpp:196):
+ * number of trial periods of a search. */
+int rt_periodogram_length(int64_t size, double tsamp, double period_min, double period_max,
+                          int64_t bins_min, int64_t bins_max, int64_t* length);
+
+/* Trial periods and fold bins of a search, host arrays of `length` elements
+ * (periodogram.hpp:190-194). */
+int rt_periodogram_grid(int64_t size, double tsamp, double period_min, double period_max,
+                        int64_t bins_min, int64_t bins_max, double* h_periods,
+                        uint32_t* h_foldbins);
+
+/* Index bounds and edge weights of downsampled sample k (downsample.hpp:58-73),
+ * computed on the host by the same exact integer emulation of the float64
+ * arithmetic that the downsampling kernels run. */
+int rt_downsample_index(int64_t k, double f, int64_t size, int64_t* imin, int64_t* imax,
+                        float* wmin, float* wmax);
+
+/* ---- Kernels of riptide.libcpp (python_bindings.cpp:225-279) -------------- */
+
+/* libcpp.ffa2 (python_bindings.cpp:84-96 -> transforms.hpp:53-61):
+ * FFA transform of a rows x cols float32 block; bit-identical output. */
+int rt_ffa2(const float* d_in, int64_t rows, int64_t cols, float* d_out, void* stream);
+
+/* libcpp.snr2 / snr1 (python_bindings.cpp:121-160 -> snr.hpp:37-65):
+ * boxcar S/N of rows x cols profiles for nw trial widths; d_out is rows x nw. */
+int rt_snr2(const float* d_in, int64_t rows, int64_t cols, const uint64_t* h_widths, int64_t nw,
+            float stdnoise, float* d_out, void* stream);
+
+/* libcpp.downsample (python_bindings.cpp:163-177 -> downsample.hpp:44-82):
+ * d_out holds floor(size / f) samples. */
+int rt_downsample(const float* d_in, int64_t size, double f, float* d_out, void* stream);
+
+/* libcpp.periodogram (python_bindings.cpp:180-209 -> periodogram.hpp:117-201):
+ * d_periods / d_foldbins / d_snrs hold length, length and length x nw values. */
+int rt_periodogram(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
+                   int64_t nw, double period_min, double period_max, int64_t bins_min,
+                   int64_t bins_max, double* d_periods, uint32_t* d_foldbins, float* d_snrs,
+                   void* stream);
+
+/* The same search run as one launch sequence per downsampling step (the
+ * "plain" schedule); identical results, kept for timing comparisons. */
+int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
+                         int64_t nw, double period_min, double period_max, int64_t bins_min,
+                         int64_t bins_max, double* d_periods, uint32_t* d_foldbins,
+                         float* d_snrs, void* stream);
+
+/* libcpp.running_median (python_bindings.cpp:212-222 -> running_median.hpp:100-132). */
+int rt_running_median(const float* d_in, int64_t size, int64_t width, float* d_out,
+                      void* stream);
+
+/* libcpp.rollback (python_bindings.cpp:44-52 -> kernels.hpp:32-38). */
+int rt_rollback(const float* d_in, int64_t size, int64_t shift, float* d_out, void* stream);
+
+/* libcpp.fused_rollback_add (python_bindings.cpp:55-70 -> kernels.hpp:19-25). */
+int rt_fused_rollback_add(const float* d_x, const float* d_y, int64_t size, int64_t shift,
+                          float* d_out, void* stream);
+
+/* libcpp.circular_prefix_sum (python_bindings.cpp:73-81 -> kernels.hpp:73-101).
+ * d_out holds nsum elements (the reference allocates only `size`). */
+int rt_circular_prefix_sum(const float* d_in, int64_t size, int64_t nsum, float* d_out,
+                           void* stream);
+
+/* libcpp.benchmark_ffa2 (python_bindings.cpp:99-118): seconds per FFA of a
+ * zero rows x cols block, timed on the device. */
+int rt_benchmark_ffa2(int64_t rows, int64_t cols, int64_t loops, double* seconds);
+
+/* ---- Python-side numerics of the path (numpy in the reference) ----------- */
+
+/* scrunch (running_medians.py:40-46): float32 means of blocks of `factor`
+ * samples; d_out holds size / factor values. */
+int rt_scrunch_mean(const float* d_in, int64_t size, int64_t factor, float* d_out, void* stream);
+
+/* TimeSeries.deredden (time_series.py:95-124 with fast_running_median,
+ * running_medians.py:49-85): d_out = data - approximate running median. */
+int rt_deredden(const float* d_in, int64_t size, int64_t width_samples, int64_t min_points,
+                float* d_out, void* stream);
+
+/* fast_running_median (running_medians.py:49-85) alone, as float32. */
+int rt_fast_running_median(const float* d_in, int64_t size, int64_t width_samples,
+                           int64_t min_points, float* d_out, void* stream);
+
+/* TimeSeries.normalise (time_series.py:67-93): zero mean, unit variance. */
+int rt_normalise(const float* d_in, int64_t size, float* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RIPTIDE_AMD_H */

@@ -1,0 +1,68 @@
+"""Builds the native library ``riptide_amd/libriptide_amd.so`` in-tree.
+
+HIP sources are compiled for gfx950 with hipcc; the search planner is compiled
+with g++ and the reference's floating-point flags (setup.py:13-20,34) so that
+its float64 plan arithmetic is bit-identical to the reference's.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libriptide_amd.so")
+BUILD = os.path.join(HERE, "_build")
+
+HIP_SOURCES = ["ffa_kernels.hip", "series_kernels.hip", "batch.hip", "adjoint.hip", "api.hip"]
+GCC_SOURCES = ["plan.cpp", "ffa_plan.cpp"]
+
+ARCH = os.environ.get("RIPTIDE_AMD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+    # packed fp32 instructions stay off (numerics)
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+GCC_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-march=x86-64-v3",
+    "-fassociative-math", "-fno-math-errno", "-ffinite-math-only",
+    "-fno-rounding-math", "-fno-signed-zeros", "-fno-trapping-math",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"build step failed: {cmd[0]} {cmd[-1]}")
+    return r
+
+
+def build(verbose=False):
+    """Compiles every source and links the shared library (always rebuilds)."""
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in HIP_SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(BUILD, src + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
+        objs.append(obj)
+    for src in GCC_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        _run(["g++", *GCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

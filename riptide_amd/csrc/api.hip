@@ -1,0 +1,684 @@
+// C-ABI entry points (include/riptide_amd.h): argument checks with the
+// reference's messages, the device scratch pool, and the launch schedules.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/riptide_amd.h"
+#include "exact_f64.h"
+#include "ffa_kernels.h"
+#include "ffa_plan.h"
+#include "plan.h"
+#include "series_kernels.h"
+#include "tasks.h"
+
+namespace rt {
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define RT_HIP(expr)                                                                  \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            return fail(RT_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// Device scratch pool: grow-only buffers per (device, slot).  Calls that use
+// it hold g_pool_mu, so one device runs one such call at a time.
+// ---------------------------------------------------------------------------
+
+enum Slot {
+    kSlotTasks = 0,
+    kSlotTables,
+    kSlotDs,
+    kSlotInter,
+    kSlotMisc,
+    kSlotErr,
+    kSlotCount
+};
+
+struct PoolBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+static std::mutex g_pool_mu;
+static std::map<int, std::vector<PoolBuf>> g_pool;
+
+static int pool_get(int slot, size_t bytes, void** out)
+{
+    int dev = 0;
+    RT_HIP(hipGetDevice(&dev));
+    auto& v = g_pool[dev];
+    if (v.empty()) v.resize(kSlotCount);
+    PoolBuf& b = v[slot];
+    if (b.bytes < bytes) {
+        if (b.p) {
+            RT_HIP(hipDeviceSynchronize());
+            RT_HIP(hipFree(b.p));
+            b.p = nullptr;
+            b.bytes = 0;
+        }
+        const size_t want = std::max<size_t>(bytes, 256);
+        RT_HIP(hipMalloc(&b.p, want));
+        b.bytes = want;
+    }
+    *out = b.p;
+    return RT_OK;
+}
+
+template <typename T>
+static int upload(int slot, const std::vector<T>& v, T** d, hipStream_t stream)
+{
+    void* p = nullptr;
+    int st = pool_get(slot, v.size() * sizeof(T), &p);
+    if (st) return st;
+    if (!v.empty())
+        RT_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    *d = static_cast<T*>(p);
+    return RT_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Argument checks (messages of the reference)
+// ---------------------------------------------------------------------------
+
+static int check_widths(const uint64_t* w, int64_t nw, int64_t bins)
+{
+    // snr.hpp:21-31
+    for (int64_t i = 0; i < nw; ++i)
+        if (!(w[i] > 0 && w[i] < (uint64_t)bins))
+            return fail(RT_EINVAL, "trial widths must be all > 0 and < columns");
+    return RT_OK;
+}
+
+static int check_stdnoise(float s)
+{
+    // snr.hpp:14-18
+    if (!(s > 0)) return fail(RT_EINVAL, "stdnoise must be > 0");
+    return RT_OK;
+}
+
+// (h + b, b) of every width, for bins in [b0, b0 + nb)
+static std::vector<float> hb_table(int64_t b0, int64_t nb, const uint64_t* w, int64_t nw)
+{
+    std::vector<float> t((size_t)(nb * nw * 2));
+    for (int64_t b = 0; b < nb; ++b)
+        for (int64_t i = 0; i < nw; ++i) {
+            float hpb = 0, bb = 0;
+            if (w[i] < (uint64_t)(b0 + b)) boxcar_hb(b0 + b, (int64_t)w[i], &hpb, &bb);
+            t[(size_t)((b * nw + i) * 2)] = hpb;
+            t[(size_t)((b * nw + i) * 2 + 1)] = bb;
+        }
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// FFA task execution
+// ---------------------------------------------------------------------------
+
+// Uploads all passes' tasks, launches them in order, waits and checks the
+// kernel's overflow flag.
+static int run_passes(const std::vector<std::vector<FfaTask>>& passes, hipStream_t stream,
+                      bool sync)
+{
+    std::vector<FfaTask> all;
+    std::vector<size_t> first;
+    for (const auto& p : passes) {
+        first.push_back(all.size());
+        all.insert(all.end(), p.begin(), p.end());
+    }
+    if (all.empty()) return RT_OK;
+    FfaTask* d_tasks = nullptr;
+    int st = upload(kSlotTasks, all, &d_tasks, stream);
+    if (st) return st;
+    void* errp = nullptr;
+    st = pool_get(kSlotErr, sizeof(int), &errp);
+    if (st) return st;
+    RT_HIP(hipMemsetAsync(errp, 0, sizeof(int), stream));
+    for (size_t i = 0; i < passes.size(); ++i)
+        RT_HIP(launch_ffa_tasks(d_tasks + first[i], (int)passes[i].size(), (int*)errp, stream));
+    if (sync) {
+        int herr = 0;
+        RT_HIP(hipMemcpyAsync(&herr, errp, sizeof(int), hipMemcpyDeviceToHost, stream));
+        RT_HIP(hipStreamSynchronize(stream));
+        if (herr) return fail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+    }
+    return RT_OK;
+}
+
+// Global-memory FFA for blocks whose rows are too wide to tile in LDS.
+static int ffa_global(const float* d_in, int64_t rows, int64_t cols, float* d_out,
+                      hipStream_t stream)
+{
+    const int H = ffa_height(rows);
+    if (H == 0) {
+        RT_HIP(hipMemcpyAsync(d_out, d_in, (size_t)(rows * cols) * sizeof(float),
+                              hipMemcpyDeviceToDevice, stream));
+        return RT_OK;
+    }
+    void* tmp = nullptr;
+    int st = pool_get(kSlotInter, (size_t)(rows * cols) * sizeof(float), &tmp);
+    if (st) return st;
+    // depth H-1 first; the last (depth 0) launch must write d_out
+    const float* src = d_in;
+    for (int d = H - 1; d >= 0; --d) {
+        const bool to_out = ((H - 1 - d) % 2) == ((H - 1) % 2);
+        float* dst = to_out ? d_out : (float*)tmp;
+        RT_HIP(launch_ffa_level_global(src, dst, rows, cols, d, stream));
+        src = dst;
+    }
+    return RT_OK;
+}
+
+static hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Minimum rows per workgroup for the LDS path.
+static constexpr int kMinCapRows = 16;
+
+}  // namespace rt
+
+using namespace rt;
+
+extern "C" {
+
+const char* rt_last_error(void) { return g_err.c_str(); }
+
+int rt_version(void) { return 1; }
+
+int rt_periodogram_length(int64_t size, double tsamp, double pmin, double pmax, int64_t bmin,
+                          int64_t bmax, int64_t* length)
+{
+    SeriesPlan p;
+    const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &p);
+    if (!e.empty()) return fail(RT_EINVAL, e);
+    *length = p.length;
+    return RT_OK;
+}
+
+int rt_periodogram_grid(int64_t size, double tsamp, double pmin, double pmax, int64_t bmin,
+                        int64_t bmax, double* h_periods, uint32_t* h_foldbins)
+{
+    SeriesPlan p;
+    const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &p);
+    if (!e.empty()) return fail(RT_EINVAL, e);
+    plan_periods(p, h_periods, h_foldbins);
+    return RT_OK;
+}
+
+int rt_downsample_index(int64_t k, double f, int64_t size, int64_t* imin, int64_t* imax,
+                        float* wmin, float* wmax)
+{
+    if (!(f > 0)) return fail(RT_EINVAL, "f must be > 0");
+    uint64_t mf;
+    int ef;
+    split_double(f, &mf, &ef);
+    const DsIndex ix = ds_index(k, mf, ef, size);
+    *imin = ix.imin;
+    *imax = ix.imax;
+    *wmin = ix.wmin;
+    *wmax = ix.wmax;
+    return RT_OK;
+}
+
+int rt_ffa2(const float* d_in, int64_t rows, int64_t cols, float* d_out, void* stream)
+{
+    if (rows < 1 || cols < 1)
+        return fail(RT_EINVAL, "ffa2 input must have at least one row and one column");
+    if (rows > (1LL << 30) || rows * cols > (1LL << 40))
+        return fail(RT_EINVAL, "ffa2 input too large");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    const int cap = ffa_cap_rows((int)std::min<int64_t>(cols, 1 << 20), ffa_lds_bytes());
+    if (cols > (1 << 20) || cap < kMinCapRows || rows > (1LL << 31) / std::max<int64_t>(cols, 1)) {
+        const int st = ffa_global(d_in, rows, cols, d_out, s);
+        if (st) return st;
+        RT_HIP(hipStreamSynchronize(s));
+        return RT_OK;
+    }
+    TransformSpec sp;
+    sp.m = rows;
+    sp.bins = (int)cols;
+    sp.cap = cap;
+    sp.src = d_in;
+    sp.kind = 0;
+    sp.dst = d_out;
+    if (rows > cap) {
+        void* inter = nullptr;
+        const size_t n = (size_t)(rows * cols);
+        int st = pool_get(kSlotInter, 2 * n * sizeof(float), &inter);
+        if (st) return st;
+        sp.buf_a = (float*)inter;
+        sp.buf_b = (float*)inter + n;
+    }
+    std::vector<std::vector<FfaTask>> passes;
+    if (plan_transform_tasks(sp, passes) < 0) {
+        const int st = ffa_global(d_in, rows, cols, d_out, s);
+        if (st) return st;
+        RT_HIP(hipStreamSynchronize(s));
+        return RT_OK;
+    }
+    return run_passes(passes, s, true);
+}
+
+int rt_snr2(const float* d_in, int64_t rows, int64_t cols, const uint64_t* h_widths, int64_t nw,
+            float stdnoise, float* d_out, void* stream)
+{
+    // python_bindings.cpp:131-132: stdnoise is checked before the widths
+    int st = check_stdnoise(stdnoise);
+    if (st) return st;
+    st = check_widths(h_widths, nw, cols);
+    if (st) return st;
+    if (rows == 0 || nw == 0) return RT_OK;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    std::vector<int32_t> w32(h_widths, h_widths + nw);
+    std::vector<float> hb = hb_table(cols, 1, h_widths, nw);
+    std::vector<float> tab(hb);
+    const size_t woff = tab.size();
+    tab.resize(woff + (size_t)nw);
+    std::memcpy(tab.data() + woff, w32.data(), (size_t)nw * sizeof(int32_t));
+    float* d_tab = nullptr;
+    st = upload(kSlotTables, tab, &d_tab, s);
+    if (st) return st;
+    float* scratch = nullptr;
+    if (cols > kWave * kSnrRegsMaxK) {
+        void* p = nullptr;
+        st = pool_get(kSlotInter, (size_t)(rows * cols) * sizeof(float), &p);
+        if (st) return st;
+        scratch = (float*)p;
+    }
+    RT_HIP(launch_snr_rows(d_in, rows, (int)cols, (const int32_t*)(d_tab + woff), d_tab, (int)nw,
+                           stdnoise, d_out, scratch, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_downsample(const float* d_in, int64_t size, double f, float* d_out, void* stream)
+{
+    // downsample.hpp:11-16
+    if (!((f > 1.0) & (f <= (double)(size_t)size)))
+        return fail(RT_EINVAL, "Downsampling factor must verify: 1 < f <= size");
+    hipStream_t s = S(stream);
+    const int64_t n = downsampled_size(size, f);
+    RT_HIP(launch_downsample_direct(d_in, size, f, n, d_out, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_running_median(const float* d_in, int64_t size, int64_t width, float* d_out, void* stream)
+{
+    // running_median.hpp:103-107
+    if (!(width % 2)) return fail(RT_EINVAL, "width must be an odd number");
+    if (!(width < size)) return fail(RT_EINVAL, "width must be < size");
+    hipStream_t s = S(stream);
+    RT_HIP(launch_running_median(d_in, size, (int)width, d_out, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_rollback(const float* d_in, int64_t size, int64_t shift, float* d_out, void* stream)
+{
+    if (size < 1) return fail(RT_EINVAL, "size must be >= 1");
+    if (shift < 0) return fail(RT_EINVAL, "shift must be positive");
+    hipStream_t s = S(stream);
+    RT_HIP(launch_rollback(d_in, size, shift, d_out, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_fused_rollback_add(const float* d_x, const float* d_y, int64_t size, int64_t shift,
+                          float* d_out, void* stream)
+{
+    if (size < 1) return fail(RT_EINVAL, "size must be >= 1");
+    if (shift < 0) return fail(RT_EINVAL, "shift must be positive");
+    hipStream_t s = S(stream);
+    RT_HIP(launch_fused_rollback_add(d_x, d_y, size, shift, d_out, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_circular_prefix_sum(const float* d_in, int64_t size, int64_t nsum, float* d_out,
+                           void* stream)
+{
+    if (size < 1) return fail(RT_EINVAL, "size must be >= 1");
+    if (nsum < 0) return fail(RT_EINVAL, "nsum must be >= 0");
+    if (nsum == 0) return RT_OK;
+    hipStream_t s = S(stream);
+    RT_HIP(launch_circular_prefix_sum(d_in, size, nsum, d_out, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_benchmark_ffa2(int64_t rows, int64_t cols, int64_t loops, double* seconds)
+{
+    if (rows < 1 || cols < 1 || loops < 1)
+        return fail(RT_EINVAL, "rows, cols and loops must be >= 1");
+    float* in = nullptr;
+    float* out = nullptr;
+    const size_t n = (size_t)(rows * cols);
+    RT_HIP(hipMalloc(&in, n * sizeof(float)));
+    RT_HIP(hipMalloc(&out, n * sizeof(float)));
+    RT_HIP(hipMemset(in, 0, n * sizeof(float)));
+    int st = rt_ffa2(in, rows, cols, out, nullptr);  // warm-up
+    hipEvent_t a, b;
+    RT_HIP(hipEventCreate(&a));
+    RT_HIP(hipEventCreate(&b));
+    RT_HIP(hipEventRecord(a, nullptr));
+    for (int64_t i = 0; i < loops && st == RT_OK; ++i)
+        st = rt_ffa2(in, rows, cols, out, nullptr);
+    RT_HIP(hipEventRecord(b, nullptr));
+    RT_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    RT_HIP(hipEventElapsedTime(&ms, a, b));
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    hipFree(in);
+    hipFree(out);
+    if (st) return st;
+    *seconds = ms * 1e-3 / (double)loops;
+    return RT_OK;
+}
+
+int rt_scrunch_mean(const float* d_in, int64_t size, int64_t factor, float* d_out, void* stream)
+{
+    if (factor < 1) return fail(RT_EINVAL, "factor must be >= 1");
+    hipStream_t s = S(stream);
+    const int64_t nb = size / factor;
+    if (nb > 0) RT_HIP(launch_scrunch_mean(d_in, nb, factor, d_out, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+static int fast_rmed(const float* d_in, int64_t size, int64_t width_samples, int64_t min_points,
+                     float* d_out, bool subtract, hipStream_t s)
+{
+    // running_medians.py:73-85
+    if (!(min_points % 2)) return fail(RT_EINVAL, "min_points must be an odd number");
+    const int64_t sf = (int64_t)std::max(1.0, width_samples / (double)min_points);
+    if (sf == 1) {
+        if (!(width_samples % 2)) return fail(RT_EINVAL, "width must be an odd number");
+        if (!(width_samples < size)) return fail(RT_EINVAL, "width must be < size");
+        if (!subtract) {
+            RT_HIP(launch_running_median(d_in, size, (int)width_samples, d_out, s));
+            return RT_OK;
+        }
+        void* p = nullptr;
+        int st = pool_get(kSlotMisc, (size_t)size * sizeof(float), &p);
+        if (st) return st;
+        RT_HIP(launch_running_median(d_in, size, (int)width_samples, (float*)p, s));
+        // data - rmed with rmed at full resolution: the interpolation kernel
+        // with sf = 1 reduces to a plain float32 subtraction
+        RT_HIP(launch_deredden_subtract(d_in, size, (const float*)p, size, 1, d_out, s));
+        return RT_OK;
+    }
+    const int64_t nl = size / sf;
+    if (!(min_points < nl)) return fail(RT_EINVAL, "width must be < size");
+    void* p = nullptr;
+    int st = pool_get(kSlotMisc, (size_t)(2 * nl) * sizeof(float), &p);
+    if (st) return st;
+    float* lores = (float*)p;
+    float* rmed = lores + nl;
+    RT_HIP(launch_scrunch_mean(d_in, nl, sf, lores, s));
+    RT_HIP(launch_running_median(lores, nl, (int)min_points, rmed, s));
+    RT_HIP(launch_deredden_subtract(subtract ? d_in : nullptr, size, rmed, nl, sf, d_out, s));
+    return RT_OK;
+}
+
+int rt_deredden(const float* d_in, int64_t size, int64_t width_samples, int64_t min_points,
+                float* d_out, void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    int st = fast_rmed(d_in, size, width_samples, min_points, d_out, true, s);
+    if (st) return st;
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_fast_running_median(const float* d_in, int64_t size, int64_t width_samples,
+                           int64_t min_points, float* d_out, void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    int st = fast_rmed(d_in, size, width_samples, min_points, d_out, false, s);
+    if (st) return st;
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_normalise(const float* d_in, int64_t size, float* d_out, void* stream)
+{
+    if (size < 1) return fail(RT_EINVAL, "cannot normalise an empty series");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    void* p = nullptr;
+    int st = pool_get(kSlotMisc, (size_t)(2 * (kNormaliseParts + 2)) * sizeof(float), &p);
+    if (st) return st;
+    RT_HIP(launch_normalise(d_in, size, d_out, (float*)p, kNormaliseParts, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Periodogram
+// ---------------------------------------------------------------------------
+
+namespace rt {
+
+struct PgramSetup {
+    SeriesPlan plan;
+    float* d_hb = nullptr;
+    int32_t* d_w = nullptr;
+    float* d_ds = nullptr;
+    int64_t nw = 0;
+};
+
+// Plans the search, checks the widths, uploads the period grid and the S/N
+// tables and runs the packed downsampling of every step.
+static int pgram_setup(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
+                       int64_t nw, double pmin, double pmax, int64_t bmin, int64_t bmax,
+                       double* d_periods, uint32_t* d_foldbins, hipStream_t s, PgramSetup* out)
+{
+    const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &out->plan);
+    if (!e.empty()) return fail(RT_EINVAL, e);
+    const SeriesPlan& p = out->plan;
+    int st = check_widths(h_widths, nw, bmin);
+    if (st) return st;
+    out->nw = nw;
+
+    std::vector<double> periods((size_t)p.length);
+    std::vector<uint32_t> foldbins((size_t)p.length);
+    plan_periods(p, periods.data(), foldbins.data());
+    if (p.length > 0) {
+        RT_HIP(hipMemcpyAsync(d_periods, periods.data(), periods.size() * sizeof(double),
+                              hipMemcpyHostToDevice, s));
+        RT_HIP(hipMemcpyAsync(d_foldbins, foldbins.data(), foldbins.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, s));
+    }
+    std::vector<float> tab = hb_table(bmin, bmax - bmin + 1, h_widths, nw);
+    const size_t woff = tab.size();
+    tab.resize(woff + (size_t)nw);
+    for (int64_t i = 0; i < nw; ++i) {
+        const int32_t w = (int32_t)h_widths[i];
+        std::memcpy(&tab[woff + (size_t)i], &w, sizeof(int32_t));
+    }
+    float* d_tab = nullptr;
+    st = upload(kSlotTables, tab, &d_tab, s);
+    if (st) return st;
+    out->d_hb = d_tab;
+    out->d_w = (int32_t*)(d_tab + woff);
+
+    // packed downsampling of every f > 1 step
+    void* dsp = nullptr;
+    st = pool_get(kSlotDs, (size_t)std::max<int64_t>(p.ds_total, 1) * sizeof(float), &dsp);
+    if (st) return st;
+    out->d_ds = (float*)dsp;
+    std::vector<DsStepDev> steps;
+    double fmax = 1.0;
+    for (const Step& stp : p.steps) {
+        if (stp.ds_offset < 0 || stp.n == 0) continue;
+        if (stp.f <= kDsPackedMaxF) {
+            DsStepDev d;
+            split_double(stp.f, &d.mf, &d.ef);
+            d.inv_f = (float)(1.0 / stp.f);
+            d.n = stp.n;
+            d.out_offset = stp.ds_offset;
+            steps.push_back(d);
+            fmax = std::max(fmax, stp.f);
+        } else {
+            RT_HIP(launch_downsample_direct(d_in, size, stp.f, stp.n, out->d_ds + stp.ds_offset, s));
+        }
+    }
+    if (!steps.empty()) {
+        if (steps.size() > (size_t)kDsMaxSteps)
+            return fail(RT_EINVAL, "too many downsampling steps");
+        const int chunk = kDsChunk;
+        const int halo = (int)std::ceil(fmax) + 2;
+        const int nchunks = (int)((size + chunk - 1) / chunk);
+        DsSeriesDev sd;
+        sd.x = d_in;
+        sd.out = out->d_ds;
+        sd.N = size;
+        sd.step0 = 0;
+        sd.nsteps = (int32_t)steps.size();
+        sd.chunk_offset = 0;
+        sd.pad = 0;
+        // the descriptors ride in the task slot's neighbour: a misc upload
+        std::vector<uint8_t> blob(sizeof(DsSeriesDev) + steps.size() * sizeof(DsStepDev));
+        std::memcpy(blob.data(), &sd, sizeof(sd));
+        std::memcpy(blob.data() + sizeof(sd), steps.data(), steps.size() * sizeof(DsStepDev));
+        uint8_t* d_blob = nullptr;
+        st = upload(kSlotMisc, blob, &d_blob, s);
+        if (st) return st;
+        RT_HIP(launch_downsample_packed((const DsSeriesDev*)d_blob, 1,
+                                        (const DsStepDev*)(d_blob + sizeof(DsSeriesDev)), nchunks,
+                                        chunk, halo, s));
+    }
+    // the host staging vectors above die with this frame
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+// Builds the tasks of every transform of steps [s0, s1) into `passes`, with
+// intermediates carved from `inter` (returns floats used).
+static int64_t pgram_tasks(const PgramSetup& su, const float* d_in, int s0, int s1, float* d_snrs,
+                           float* inter, std::vector<std::vector<FfaTask>>& passes, bool dry)
+{
+    const SeriesPlan& p = su.plan;
+    int64_t used = 0;
+    for (int is = s0; is < s1; ++is) {
+        const Step& stp = p.steps[is];
+        const float* src = (stp.ds_offset < 0) ? d_in : su.d_ds + stp.ds_offset;
+        for (int it = stp.first_tr; it < stp.first_tr + stp.ntr; ++it) {
+            const Transform& t = p.tr[it];
+            if (t.rows_eval <= 0) continue;
+            const int cap = ffa_cap_rows(t.bins, ffa_lds_bytes());
+            const int64_t n = (int64_t)t.rows * t.bins;
+            TransformSpec sp;
+            sp.m = t.rows;
+            sp.bins = t.bins;
+            sp.cap = cap;
+            sp.src = src;
+            sp.kind = 1;
+            sp.snr = d_snrs + t.out_row * su.nw;
+            sp.widths = su.d_w;
+            sp.hb = su.d_hb + (size_t)(t.bins - p.bmin) * su.nw * 2;
+            sp.nw = (int)su.nw;
+            sp.stdnoise = t.stdnoise;
+            sp.rows_eval = t.rows_eval;
+            if (t.rows > cap) {
+                if (!dry) {
+                    sp.buf_a = inter + used;
+                    sp.buf_b = inter + used + n;
+                }
+                used += 2 * n;
+            }
+            if (!dry && plan_transform_tasks(sp, passes) < 0) return -1;
+        }
+    }
+    return used;
+}
+
+}  // namespace rt
+
+extern "C" {
+
+int rt_periodogram(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
+                   int64_t nw, double pmin, double pmax, int64_t bmin, int64_t bmax,
+                   double* d_periods, uint32_t* d_foldbins, float* d_snrs, void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    PgramSetup su;
+    int st = pgram_setup(d_in, size, tsamp, h_widths, nw, pmin, pmax, bmin, bmax, d_periods,
+                         d_foldbins, s, &su);
+    if (st) return st;
+    if (su.plan.length == 0 || nw == 0) {
+        RT_HIP(hipStreamSynchronize(s));
+        return RT_OK;
+    }
+    // every step at once: pass k of all transforms in one launch
+    std::vector<std::vector<FfaTask>> passes;
+    const int ns = (int)su.plan.steps.size();
+    const int64_t need = pgram_tasks(su, d_in, 0, ns, d_snrs, nullptr, passes, true);
+    void* inter = nullptr;
+    st = pool_get(kSlotInter, (size_t)std::max<int64_t>(need, 1) * sizeof(float), &inter);
+    if (st) return st;
+    if (pgram_tasks(su, d_in, 0, ns, d_snrs, (float*)inter, passes, false) < 0)
+        return fail(RT_EINVAL, "bins too large for the FFA tiling");
+    return run_passes(passes, s, true);
+}
+
+int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
+                         int64_t nw, double pmin, double pmax, int64_t bmin, int64_t bmax,
+                         double* d_periods, uint32_t* d_foldbins, float* d_snrs, void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    PgramSetup su;
+    int st = pgram_setup(d_in, size, tsamp, h_widths, nw, pmin, pmax, bmin, bmax, d_periods,
+                         d_foldbins, s, &su);
+    if (st) return st;
+    if (su.plan.length == 0 || nw == 0) {
+        RT_HIP(hipStreamSynchronize(s));
+        return RT_OK;
+    }
+    const int ns = (int)su.plan.steps.size();
+    int64_t need = 0;
+    for (int is = 0; is < ns; ++is) {
+        std::vector<std::vector<FfaTask>> dummy;
+        need = std::max(need, pgram_tasks(su, d_in, is, is + 1, d_snrs, nullptr, dummy, true));
+    }
+    void* inter = nullptr;
+    st = pool_get(kSlotInter, (size_t)std::max<int64_t>(need, 1) * sizeof(float), &inter);
+    if (st) return st;
+    // one launch sequence per step: tasks of all steps are uploaded at once,
+    // each step's passes launched in order, steps in order
+    std::vector<std::vector<std::vector<FfaTask>>> per_step(ns);
+    std::vector<FfaTask> all;
+    for (int is = 0; is < ns; ++is)
+        if (pgram_tasks(su, d_in, is, is + 1, d_snrs, (float*)inter, per_step[is], false) < 0)
+            return fail(RT_EINVAL, "bins too large for the FFA tiling");
+    std::vector<std::vector<FfaTask>> flat;
+    for (auto& ps : per_step)
+        for (auto& p : ps) flat.push_back(p);
+    return run_passes(flat, s, true);
+}
+
+}  // extern "C"

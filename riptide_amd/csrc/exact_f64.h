@@ -1,0 +1,144 @@
+// Exact integer emulation of the few float64 operations the downsampling
+// index math needs (downsample.hpp:58-73): k*f and k*f + f rounded to float64
+// (round-to-nearest-even), their floor, and the float32 edge weights.
+//
+// A float64 factor f is passed as an integer significand and exponent,
+// f = mf * 2^ef with 2^52 <= mf < 2^53, so no float64 arithmetic runs on the
+// device; results are bit-identical to the reference's double arithmetic.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rt {
+
+typedef unsigned __int128 u128;
+
+// value = q * 2^e (q == 0 means zero)
+struct Dbl {
+    uint64_t q;
+    int e;
+};
+
+__host__ __device__ inline int bitlen128(u128 x)
+{
+    const uint64_t hi = (uint64_t)(x >> 64);
+    const uint64_t lo = (uint64_t)x;
+    if (hi) return 128 - __builtin_clzll(hi);
+    if (lo) return 64 - __builtin_clzll(lo);
+    return 0;
+}
+
+// Rounds the exact value P * 2^e to a float64 (53-bit significand, RNE).
+__host__ __device__ inline Dbl round53(u128 P, int e)
+{
+    Dbl r;
+    if (P == 0) {
+        r.q = 0;
+        r.e = 0;
+        return r;
+    }
+    const int nb = bitlen128(P);
+    if (nb <= 53) {
+        r.q = (uint64_t)P;
+        r.e = e;
+        return r;
+    }
+    const int sh = nb - 53;
+    u128 q = P >> sh;
+    const u128 rem = P - (q << sh);
+    const u128 half = (u128)1 << (sh - 1);
+    int ee = e + sh;
+    if (rem > half || (rem == half && (q & 1))) {
+        q += 1;
+        if (q >> 53) {
+            q >>= 1;
+            ee += 1;
+        }
+    }
+    r.q = (uint64_t)q;
+    r.e = ee;
+    return r;
+}
+
+// fl(k * f)
+__host__ __device__ inline Dbl dmul_int(uint64_t k, uint64_t mf, int ef)
+{
+    return round53((u128)k * mf, ef);
+}
+
+// fl(a + f), a >= 0, f > 0
+__host__ __device__ inline Dbl dadd(Dbl a, uint64_t mf, int ef)
+{
+    if (a.q == 0) {
+        Dbl r;
+        r.q = mf;
+        r.e = ef;
+        return r;
+    }
+    const int emin = a.e < ef ? a.e : ef;
+    const u128 s = ((u128)a.q << (a.e - emin)) + ((u128)mf << (ef - emin));
+    return round53(s, emin);
+}
+
+// floor of a non-negative value
+__host__ __device__ inline int64_t dfloor(Dbl a)
+{
+    if (a.q == 0) return 0;
+    if (a.e >= 0) return (int64_t)(a.q << a.e);
+    if (a.e <= -64) return 0;
+    return (int64_t)(a.q >> (-a.e));
+}
+
+// float32(i - a) where the float64 difference i - a is exact, 0 <= i - a
+__host__ __device__ inline float dsub_to_float(int64_t i, Dbl a)
+{
+    if (a.q == 0) return (float)i;
+    if (a.e >= 0) {
+        // a is an integer
+        return (float)(i - (int64_t)(a.q << a.e));
+    }
+    const int s = -a.e;
+    const u128 num = ((u128)(uint64_t)i << s) - (u128)a.q;  // in units of 2^-s
+    // num < 2^(s+2) <= 2^55: exact in uint64
+    const float v = (float)(uint64_t)num;  // one rounding (RNE)
+    return ldexpf(v, -s);
+}
+
+// float32(a - i) where the float64 difference a - i is exact, a >= i
+__host__ __device__ inline float dsub_from_float(Dbl a, int64_t i)
+{
+    if (a.q == 0) return (float)(-i);
+    if (a.e >= 0) return (float)((int64_t)(a.q << a.e) - i);
+    const int s = -a.e;
+    const u128 num = (u128)a.q - ((u128)(uint64_t)i << s);
+    const float v = (float)(uint64_t)num;
+    return ldexpf(v, -s);
+}
+
+// Per-output index math of downsample.hpp:58-73.
+struct DsIndex {
+    int64_t imin, imax;
+    float wmin, wmax;
+};
+
+__host__ __device__ inline DsIndex ds_index(int64_t k, uint64_t mf, int ef, int64_t N)
+{
+    const Dbl start = dmul_int((uint64_t)k, mf, ef);
+    const Dbl end = dadd(start, mf, ef);
+    DsIndex r;
+    r.imin = dfloor(start);
+    const int64_t fe = dfloor(end);
+    r.imax = fe < N - 1 ? fe : N - 1;
+    r.wmin = dsub_to_float(r.imin + 1, start);
+    r.wmax = dsub_from_float(end, r.imax);
+    return r;
+}
+
+// imin(k) = floor(fl(k*f))
+__host__ __device__ inline int64_t ds_imin(int64_t k, uint64_t mf, int ef)
+{
+    return dfloor(dmul_int((uint64_t)k, mf, ef));
+}
+
+}  // namespace rt

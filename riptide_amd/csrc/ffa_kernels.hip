@@ -1,0 +1,477 @@
+// FFA transform + boxcar S/N kernels for gfx950.
+//
+// One workgroup (16 wavefronts) computes one FfaTask: it stages the rows a
+// tile of the merge tree depends on in LDS, applies the tree's merge levels
+// in place (every level is read into registers, then written back after a
+// barrier), and either writes the finished rows back to HBM or turns each
+// finished row into per-width boxcar S/N without the row ever leaving LDS.
+//
+// Merge order and row map follow transforms.hpp:13-50: node of M rows =
+// head (M>>1 rows) + tail; output row s adds head row h(s) and tail row t(s)
+// rolled left by s - t(s), where h and t are computed in float32 as
+// trunc(k*s + 0.5f) with k = (rows_child - 1)/(M - 1).  Every output element
+// is one float32 add of the same two operands as the reference, so the
+// transform is bit-identical.
+#include <hip/hip_runtime.h>
+
+#include "ffa_device.h"
+#include "ffa_kernels.h"
+#include "ffa_plan.h"
+#include "tasks.h"
+
+namespace rt {
+
+// LDS layout of the FFA task kernel (bytes are dynamic; see ffa_lds_bytes()):
+//   float data[cap_rows * bins]      staged rows
+//   int   tab_a[cap_rows]            per-row head (or copy) offset, in floats
+//   int   tab_b[cap_rows]            per-row tail offset, -1 for a copy
+//   int   tab_s[cap_rows]            per-row circular shift
+//   Seg   seg[2^(kMaxConeLevels+1)]  cone segments (cone tasks only)
+struct Seg {
+    int m;     // node rows (0: empty slot)
+    int r0;    // node first row (transform-relative)
+    int lo;    // first needed row (node-relative)
+    int len;   // number of needed rows
+    int base;  // first LDS row of this segment in its level's layout
+};
+
+__device__ __forceinline__ int seg_slot(int t, int j) { return (1 << t) - 1 + j; }
+
+// Finds the segment slot of level t whose LDS range holds row p.
+__device__ __forceinline__ int seg_find(const Seg* seg, int t, int p)
+{
+    const int first = (1 << t) - 1;
+    int lo = 0, hi = (1 << t) - 1;
+    // last slot with base <= p and len > 0; bases are non-decreasing
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (seg[first + mid].base <= p)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    // empty slots share the base of the next non-empty one; step back to the
+    // slot that really contains p
+    while (lo > 0 && (seg[first + lo].len == 0 || seg[first + lo].base + seg[first + lo].len <= p))
+        --lo;
+    return first + lo;
+}
+
+// Builds the cone of a tile top-down: level 0 is the tile itself; level t+1
+// holds the rows of the children that level t reads.  Returns R_levels (rows
+// at the bottom) in *bottom_rows.  Must be called by the whole workgroup.
+__device__ void build_cone(Seg* seg, const FfaTask& tk, int* shared_int)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        Seg s;
+        s.m = tk.node_m;
+        s.r0 = tk.node_r0;
+        s.lo = tk.lo;
+        s.len = tk.hi - tk.lo;
+        s.base = 0;
+        seg[0] = s;
+    }
+    __syncthreads();
+    for (int t = 0; t < tk.levels; ++t) {
+        const int nslots = 1 << t;
+        for (int j = tid; j < nslots; j += blockDim.x) {
+            const Seg p = seg[seg_slot(t, j)];
+            Seg a, b;
+            a.m = 0; a.r0 = p.r0; a.lo = 0; a.len = 0; a.base = 0;
+            b = a;
+            if (p.len > 0 && p.m == 1) {
+                a = p;  // single-row leaf: carried down unchanged
+            } else if (p.len > 0 && p.m >= 2) {
+                const int mh = p.m >> 1;
+                const int mt = p.m - mh;
+                const float kh = ((float)mh - 1.0f) / ((float)p.m - 1.0f);
+                const float kt = ((float)mt - 1.0f) / ((float)p.m - 1.0f);
+                const int s0 = p.lo, s1 = p.lo + p.len - 1;
+                const int h0 = merge_row(kh, s0), h1 = merge_row(kh, s1);
+                const int t0 = merge_row(kt, s0), t1 = merge_row(kt, s1);
+                a.m = mh; a.r0 = p.r0; a.lo = h0; a.len = h1 - h0 + 1;
+                b.m = mt; b.r0 = p.r0 + mh; b.lo = t0; b.len = t1 - t0 + 1;
+            }
+            seg[seg_slot(t + 1, 2 * j)] = a;
+            seg[seg_slot(t + 1, 2 * j + 1)] = b;
+        }
+        __syncthreads();
+        // exclusive scan of the children's lengths (<= 2^kMaxConeLevels slots)
+        if (tid < kWave) {
+            const int n = 2 << t;
+            const int per = (n + kWave - 1) / kWave;
+            const int first = seg_slot(t + 1, 0);
+            int run = 0;
+            for (int q = 0; q < per; ++q) {
+                const int j = tid * per + q;
+                if (j < n) run += seg[first + j].len;
+            }
+            int incl = run;
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int o = __shfl_up(incl, d, kWave);
+                if (tid >= d) incl += o;
+            }
+            int excl = incl - run;
+            for (int q = 0; q < per; ++q) {
+                const int j = tid * per + q;
+                if (j < n) {
+                    seg[first + j].base = excl;
+                    excl += seg[first + j].len;
+                }
+            }
+            if (tid == kWave - 1) *shared_int = incl;
+        }
+        __syncthreads();
+    }
+}
+
+// Row map of one merge level of a whole-node (identity layout) task: row p of
+// the node at relative depth t.
+__device__ __forceinline__ void identity_row_map(int node_m, int t, int p, int bins, int* a,
+                                                 int* b, int* sh)
+{
+    int r0 = 0, m = node_m;
+    for (int d = 0; d < t && m > 1; ++d) {
+        const int mh = m >> 1;
+        if (p < r0 + mh) {
+            m = mh;
+        } else {
+            r0 += mh;
+            m -= mh;
+        }
+    }
+    if (m <= 1) {
+        *a = p * bins;
+        *b = -1;
+        *sh = 0;
+        return;
+    }
+    const int mh = m >> 1;
+    const int mt = m - mh;
+    const float kh = ((float)mh - 1.0f) / ((float)m - 1.0f);
+    const float kt = ((float)mt - 1.0f) / ((float)m - 1.0f);
+    const int s = p - r0;
+    const int h = merge_row(kh, s);
+    const int tt = merge_row(kt, s);
+    *a = (r0 + h) * bins;
+    *b = (r0 + mh + tt) * bins;
+    *sh = (s - tt) % bins;
+}
+
+// Row map of one merge level of a cone task (level t reads level t+1).
+__device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int bins, int* a,
+                                             int* b, int* sh)
+{
+    const int slot = seg_find(seg, t, p);
+    const Seg g = seg[slot];
+    const int j = slot - ((1 << t) - 1);
+    const Seg ca = seg[seg_slot(t + 1, 2 * j)];
+    const Seg cb = seg[seg_slot(t + 1, 2 * j + 1)];
+    const int s = g.lo + (p - g.base);
+    if (g.m == 1) {
+        *a = (ca.base + (s - ca.lo)) * bins;
+        *b = -1;
+        *sh = 0;
+        return;
+    }
+    const int mh = g.m >> 1;
+    const int mt = g.m - mh;
+    const float kh = ((float)mh - 1.0f) / ((float)g.m - 1.0f);
+    const float kt = ((float)mt - 1.0f) / ((float)g.m - 1.0f);
+    const int h = merge_row(kh, s);
+    const int tt = merge_row(kt, s);
+    *a = (ca.base + (h - ca.lo)) * bins;
+    *b = (cb.base + (tt - cb.lo)) * bins;
+    *sh = (s - tt) % bins;
+}
+
+template <int MAXE>
+__global__ __launch_bounds__(kFfaThreads) void ffa_task_kernel(const FfaTask* __restrict__ tasks,
+                                                                int ntasks, int* err)
+{
+    extern __shared__ float lds[];
+    const int task_id = blockIdx.x;
+    if (task_id >= ntasks) return;
+    const FfaTask tk = tasks[task_id];
+    const int bins = tk.bins;
+    const int tid = threadIdx.x;
+    const int cap_rows = tk.pad;
+
+    float* data = lds;
+    int* tab_a = reinterpret_cast<int*>(data + (size_t)cap_rows * bins);
+    int* tab_b = tab_a + cap_rows;
+    int* tab_s = tab_b + cap_rows;
+    Seg* seg = reinterpret_cast<Seg*>(tab_s + cap_rows);
+    __shared__ int s_bottom;
+
+    // ---- rows needed at the bottom of the task, and where they come from
+    int rows;  // rows in the current level's layout
+    if (tk.cone) {
+        build_cone(seg, tk, &s_bottom);
+        rows = (tk.levels > 0) ? s_bottom : (tk.hi - tk.lo);
+        if (rows > cap_rows) {
+            // the host sizes tiles so this cannot happen; never overrun LDS
+            if (tid == 0) atomicOr(err, 1);
+            return;
+        }
+        // global source row of every bottom row
+        const int t = tk.levels;
+        for (int p = tid; p < rows; p += blockDim.x) {
+            const int slot = seg_find(seg, t, p);
+            const Seg g = seg[slot];
+            tab_a[p] = (g.r0 + g.lo + (p - g.base)) * bins;
+        }
+        __syncthreads();
+        const int total = rows * bins;
+        for (int e = tid; e < total; e += blockDim.x) {
+            const int p = e / bins;
+            const int c = e - p * bins;
+            data[e] = tk.src[(size_t)tab_a[p] + c];
+        }
+    } else {
+        rows = tk.node_m;
+        const float* src = tk.src + (size_t)tk.node_r0 * bins;
+        const int total = rows * bins;
+        for (int e = tid; e < total; e += blockDim.x)
+            data[e] = src[e];
+    }
+
+    // ---- merge levels, bottom-up, in place through registers
+    for (int t = tk.levels - 1; t >= 0; --t) {
+        int out_rows;
+        if (tk.cone) {
+            out_rows = 0;
+            const int first = seg_slot(t, 0);
+            const int n = 1 << t;
+            // rows of level t = sum of segment lengths (last base + len)
+            out_rows = seg[first + n - 1].base + seg[first + n - 1].len;
+        } else {
+            out_rows = rows;
+        }
+        // table of level t
+        for (int p = tid; p < out_rows; p += blockDim.x) {
+            int a, b, sh;
+            if (tk.cone)
+                cone_row_map(seg, t, p, bins, &a, &b, &sh);
+            else
+                identity_row_map(tk.node_m, t, p, bins, &a, &b, &sh);
+            tab_a[p] = a;
+            tab_b[p] = b;
+            tab_s[p] = sh;
+        }
+        __syncthreads();
+        // compute into registers
+        float v[MAXE];
+        const int total = out_rows * bins;
+        {
+            int p = tid / bins;
+            int c = tid - p * bins;
+            // the walk below is the same on every level; keep the compiler
+            // from hoisting all MAXE (row, column) pairs out of the level loop
+            asm volatile("" : "+v"(p), "+v"(c));
+            const int dp = blockDim.x / bins;
+            const int dc = blockDim.x - dp * bins;
+#pragma unroll
+            for (int q = 0; q < MAXE; ++q) {
+                const int e = tid + q * blockDim.x;
+                if (e < total) {
+                    const int a = tab_a[p];
+                    const int b = tab_b[p];
+                    int cc = c + tab_s[p];
+                    cc = (cc >= bins) ? cc - bins : cc;
+                    const float x = data[a + c];
+                    const float y = (b >= 0) ? data[(b >= 0 ? b : 0) + cc] : -0.0f;
+                    v[q] = x + y;
+                }
+                p += dp;
+                c += dc;
+                if (c >= bins) {
+                    c -= bins;
+                    ++p;
+                }
+                // keep at most a few iterations' loads in flight per wave so
+                // that the 40 staged values, not the addresses, own the VGPRs
+                if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < MAXE; ++q) {
+            const int e = tid + q * blockDim.x;
+            if (e < total) data[e] = v[q];
+        }
+        __syncthreads();
+        rows = out_rows;
+    }
+
+    // ---- outputs
+    if (tk.kind == 0) {
+        float* dst = tk.dst + ((size_t)tk.node_r0 + tk.lo) * bins;
+        const int total = rows * bins;
+        for (int e = tid; e < total; e += blockDim.x)
+            dst[e] = data[e];
+        return;
+    }
+
+    // S/N of every finished row s = lo + p with s < rows_eval, one wavefront
+    // per row, rows processed in lockstep rounds (uniform barrier count).
+    const int wave = tid / kWave;
+    const int nwaves = blockDim.x / kWave;
+    const int lane = tid & (kWave - 1);
+    const int s_end = min(tk.hi, tk.rows_eval);
+    const int nrows = max(0, s_end - tk.lo);
+    const int rounds = (nrows + nwaves - 1) / nwaves;
+    for (int r = 0; r < rounds; ++r) {
+        const int p = r * nwaves + wave;
+        const bool active = p < nrows;
+        float* row = data + (size_t)(active ? p : 0) * bins;
+        float* out = tk.snr + (size_t)(tk.lo + p) * tk.nw;
+        if (bins <= kWave * kSnrFfaK) {
+            float own[kSnrFfaK];
+            if (active)
+                snr_prefix_inplace<kSnrFfaK>(row, bins, lane, own);
+            __syncthreads();
+            if (active)
+                snr_widths<kSnrFfaK>(row, bins, lane, own, tk.widths, tk.hb, tk.nw, tk.stdnoise,
+                                     out);
+        } else {
+            if (active)
+                snr_prefix_long(row, row, bins, lane);
+            __syncthreads();
+            if (active)
+                snr_widths_long(row, bins, lane, tk.widths, tk.hb, tk.nw, tk.stdnoise, out);
+        }
+        __syncthreads();
+    }
+}
+
+// Plain S/N of rows x cols profiles (python_bindings.cpp:121-160): one
+// wavefront per row, the row staged in LDS.
+__global__ __launch_bounds__(256) void snr_rows_kernel(const float* __restrict__ x, int rows,
+                                                       int cols, const int32_t* widths,
+                                                       const float* hb, int nw,
+                                                       float stdnoise, float* __restrict__ out)
+{
+    extern __shared__ float lds[];
+    const int wave = threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int nwaves = blockDim.x / kWave;
+    const int r = blockIdx.x * nwaves + wave;
+    float* row = lds + (size_t)wave * cols;
+    if (r < rows) {
+        const float* src = x + (size_t)r * cols;
+        for (int c = lane; c < cols; c += kWave)
+            row[c] = src[c];
+    }
+    float own[kSnrMaxK];
+    __syncthreads();
+    if (r < rows)
+        snr_prefix_inplace<kSnrMaxK>(row, cols, lane, own);
+    __syncthreads();
+    if (r < rows)
+        snr_widths<kSnrMaxK>(row, cols, lane, own, widths, hb, nw, stdnoise, out + (size_t)r * nw);
+}
+
+// Same for rows longer than the register path supports: the prefix sums go
+// to a global scratch row, one workgroup (one wavefront) per row.
+__global__ __launch_bounds__(64) void snr_rows_long_kernel(const float* __restrict__ x, int rows,
+                                                           int cols, const int32_t* widths,
+                                                           const float* hb, int nw,
+                                                           float stdnoise, float* scratch,
+                                                           float* __restrict__ out)
+{
+    const int r = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (r >= rows) return;
+    float* cps = scratch + (size_t)r * cols;
+    snr_prefix_long(x + (size_t)r * cols, cps, cols, lane);
+    __syncthreads();
+    snr_widths_long(cps, cols, lane, widths, hb, nw, stdnoise, out + (size_t)r * nw);
+}
+
+// One FFA merge level straight from HBM for blocks too wide for LDS: every
+// output element adds the two operands the reference adds.  Launched once
+// per tree depth, deepest first.
+__global__ void ffa_level_global_kernel(const float* __restrict__ src, float* __restrict__ dst,
+                                        int64_t rows, int64_t cols, int depth)
+{
+    const int64_t total = rows * cols;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t p = e / cols;
+        const int64_t c = e - p * cols;
+        // node at `depth` containing row p
+        int64_t r0 = 0, m = rows;
+        for (int d = 0; d < depth && m > 1; ++d) {
+            const int64_t mh = m >> 1;
+            if (p < r0 + mh) m = mh;
+            else { r0 += mh; m -= mh; }
+        }
+        if (m <= 1) {
+            dst[e] = src[e];
+            continue;
+        }
+        const int64_t mh = m >> 1;
+        const int64_t mt = m - mh;
+        const float kh = ((float)mh - 1.0f) / ((float)m - 1.0f);
+        const float kt = ((float)mt - 1.0f) / ((float)m - 1.0f);
+        const int64_t s = p - r0;
+        const int64_t h = merge_row(kh, (int)s);
+        const int64_t t = merge_row(kt, (int)s);
+        int64_t cc = c + (s - t) % cols;
+        if (cc >= cols) cc -= cols;
+        dst[e] = src[(r0 + h) * cols + c] + src[(r0 + mh + t) * cols + cc];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipStream_t stream)
+{
+    static bool configured = false;
+    if (!configured) {
+        hipError_t e = hipFuncSetAttribute((const void*)ffa_task_kernel<kFfaMaxE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           ffa_lds_bytes());
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    if (ntasks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ffa_task_kernel<kFfaMaxE>, dim3(ntasks), dim3(kFfaThreads),
+                       ffa_lds_bytes(), stream, d_tasks, ntasks, d_err);
+    return hipGetLastError();
+}
+
+hipError_t launch_snr_rows(const float* x, int64_t rows, int cols, const int32_t* widths,
+                           const float* hb, int nw, float stdnoise, float* out,
+                           float* long_scratch, hipStream_t stream)
+{
+    if (rows <= 0) return hipSuccess;
+    if (cols <= kWave * kSnrMaxK) {
+        const int waves = 4;
+        hipLaunchKernelGGL(snr_rows_kernel, dim3((unsigned)((rows + waves - 1) / waves)),
+                           dim3(waves * kWave), (size_t)waves * cols * sizeof(float), stream, x,
+                           (int)rows, cols, widths, hb, nw, stdnoise, out);
+    } else {
+        hipLaunchKernelGGL(snr_rows_long_kernel, dim3((unsigned)rows), dim3(kWave), 0, stream, x,
+                           (int)rows, cols, widths, hb, nw, stdnoise, long_scratch, out);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ffa_level_global(const float* src, float* dst, int64_t rows, int64_t cols,
+                                   int depth, hipStream_t stream)
+{
+    const int64_t total = rows * cols;
+    int64_t g = (total + 255) / 256;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    hipLaunchKernelGGL(ffa_level_global_kernel, dim3((unsigned)g), dim3(256), 0, stream, src, dst,
+                       rows, cols, depth);
+    return hipGetLastError();
+}
+
+}  // namespace rt

@@ -1,0 +1,50 @@
+// Splits one FFA transform into LDS-sized work items (FfaTask) grouped in
+// passes: pass 0 transforms whole merge-tree nodes straight from the folded
+// input; every later pass computes tiles of rows a few merge levels further
+// up from the previous pass's state, the last one fused with the boxcar S/N
+// (or writing the transform).
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "tasks.h"
+
+namespace rt {
+
+// ceil(log2(m)): number of merge levels of an m-row node (m >= 1).
+int ffa_height(int64_t m);
+
+// Rows of `bins` columns that one workgroup can stage (see ffa_lds_layout).
+int ffa_cap_rows(int bins, int lds_bytes);
+
+// Bytes of dynamic LDS the FFA task kernel is launched with.
+int ffa_lds_bytes();
+
+struct TransformSpec {
+    int64_t m = 0;          // rows
+    int bins = 0;           // columns
+    int cap = 0;            // rows per workgroup (ffa_cap_rows)
+    const float* src = nullptr;   // folded input rows
+    float* buf_a = nullptr; // intermediate states (m*bins floats each)
+    float* buf_b = nullptr;
+    // final output: kind 0 writes the transform to dst; kind 1 computes S/N
+    int kind = 0;
+    float* dst = nullptr;
+    float* snr = nullptr;
+    const int32_t* widths = nullptr;
+    const float* hb = nullptr;
+    int nw = 0;
+    float stdnoise = 1.0f;
+    int rows_eval = 0;
+};
+
+// Appends the transform's tasks to passes[0..], growing `passes` as needed;
+// returns the number of passes used, or -1 if the transform cannot be tiled
+// in LDS (too few rows fit).
+int plan_transform_tasks(const TransformSpec& s, std::vector<std::vector<FfaTask>>& passes);
+
+// Number of passes plan_transform_tasks would use (no tasks built).
+int count_transform_passes(int64_t m, int bins, int cap);
+
+}  // namespace rt

@@ -1,0 +1,66 @@
+// Device descriptors and host launchers of the per-series kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace rt {
+
+constexpr int kDsThreads = 256;
+constexpr int kDsMaxSteps = 256;
+// Input samples staged per workgroup by the packed downsampling kernel, and
+// the largest factor it handles (its halo must fit next to the chunk in
+// 48 KiB of LDS); larger factors use the direct kernel.
+constexpr int kDsChunk = 8192;
+constexpr double kDsPackedMaxF = 4000.0;
+
+// One downsampling step: f = mf * 2^ef exactly (the float64 factor split
+// into integer significand and exponent), inv_f ~ 1/f for index estimates.
+struct DsStepDev {
+    uint64_t mf;
+    int32_t ef;
+    float inv_f;
+    int64_t n;
+    int64_t out_offset;
+};
+
+// One series of the packed multi-step downsampling launch.
+struct DsSeriesDev {
+    const float* x;
+    float* out;
+    int64_t N;
+    int32_t step0;       // first step in the step table
+    int32_t nsteps;
+    int32_t chunk_offset;  // first workgroup of this series
+    int32_t pad;
+};
+
+// Splits a positive float64 into (mf, ef) with f == mf * 2^ef, 2^52 <= mf < 2^53.
+void split_double(double f, uint64_t* mf, int* ef);
+
+// Launchers (all asynchronous on `stream`).
+hipError_t launch_downsample_direct(const float* in, int64_t N, double f, int64_t n, float* out,
+                                    hipStream_t stream);
+hipError_t launch_downsample_packed(const DsSeriesDev* d_series, int nseries,
+                                    const DsStepDev* d_steps, int total_chunks, int chunk,
+                                    int halo, hipStream_t stream);
+hipError_t launch_running_median(const float* x, int64_t N, int width, float* out,
+                                 hipStream_t stream);
+hipError_t launch_scrunch_mean(const float* x, int64_t nblocks, int64_t factor, float* out,
+                               hipStream_t stream);
+hipError_t launch_deredden_subtract(const float* x, int64_t N, const float* fp, int64_t nl,
+                                    int64_t sf, float* out, hipStream_t stream);
+// mean/std workspace: 2*(parts + 2) floats
+hipError_t launch_normalise(const float* x, int64_t N, float* out, float* work, int parts,
+                            hipStream_t stream);
+hipError_t launch_rollback(const float* x, int64_t size, int64_t shift, float* out,
+                           hipStream_t stream);
+hipError_t launch_fused_rollback_add(const float* x, const float* y, int64_t size, int64_t shift,
+                                     float* z, hipStream_t stream);
+hipError_t launch_circular_prefix_sum(const float* x, int64_t size, int64_t nsum, float* out,
+                                      hipStream_t stream);
+hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t stream);
+
+constexpr int kNormaliseParts = 1024;
+
+}  // namespace rt

@@ -1,0 +1,601 @@
+// Per-series kernels for gfx950: fractional downsampling (single factor and
+// all factors of a search in one pass), running median, the block means and
+// interpolation of the dereddening, normalisation, and the small helpers of
+// the reference's binding (rollback, fused_rollback_add, circular prefix sum).
+#include <hip/hip_runtime.h>
+
+#include "exact_f64.h"
+#include "ffa_device.h"
+#include "series_kernels.h"
+
+namespace rt {
+
+// ---------------------------------------------------------------------------
+// Downsampling (downsample.hpp:44-82)
+// ---------------------------------------------------------------------------
+
+// Sum of in[imin..imax] with the reference's edge weights; the interior is
+// accumulated left to right and the last term fused as the reference build
+// does (acc + wmax*x -> fma).
+__device__ __forceinline__ float ds_accumulate(const float* in, const DsIndex& ix)
+{
+    float acc = ix.wmin * in[ix.imin];
+    for (int64_t i = ix.imin + 1; i < ix.imax; ++i)
+        acc += in[i];
+    return __fmaf_rn(ix.wmax, in[ix.imax], acc);
+}
+
+// One thread per output sample, inputs read straight from HBM.  Used for
+// factors too large for the LDS-staged kernel.
+__global__ void downsample_direct_kernel(const float* __restrict__ in, int64_t N, uint64_t mf,
+                                         int ef, int64_t n, float* __restrict__ out)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const DsIndex ix = ds_index(k, mf, ef, N);
+        out[k] = ds_accumulate(in, ix);
+    }
+}
+
+// First output sample whose imin is >= c.
+__device__ __forceinline__ int64_t ds_first_at(int64_t c, const DsStepDev& st)
+{
+    if (c <= 0) return 0;
+    int64_t k = (int64_t)((float)c * st.inv_f);  // float estimate, fixed below
+    if (k < 0) k = 0;
+    if (k > st.n) k = st.n;
+    while (k > 0 && ds_imin(k - 1, st.mf, st.ef) >= c) --k;
+    while (k < st.n && ds_imin(k, st.mf, st.ef) < c) ++k;
+    return k;
+}
+
+// All downsampling steps of a search in one pass over the raw series: every
+// workgroup stages one chunk of the input (plus a halo of the largest factor)
+// in LDS and produces, for every step, the outputs whose first input sample
+// lies in the chunk.
+__global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
+    const DsSeriesDev* __restrict__ series, int nseries, const DsStepDev* __restrict__ steps,
+    int chunk, int halo)
+{
+    extern __shared__ float s_in[];
+    __shared__ int64_t s_kbeg[kDsMaxSteps];
+    __shared__ int s_cnt_prefix[kDsMaxSteps + 1];
+
+    // find the series of this chunk
+    int lo = 0, hi = nseries - 1;
+    const int b = blockIdx.x;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (series[mid].chunk_offset <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    const DsSeriesDev sr = series[lo];
+    const int64_t c0 = (int64_t)(b - sr.chunk_offset) * chunk;
+    const int64_t c1 = min(c0 + chunk, sr.N);
+    const int64_t l1 = min(c0 + chunk + halo, sr.N);
+    for (int64_t i = c0 + threadIdx.x; i < l1; i += blockDim.x)
+        s_in[i - c0] = sr.x[i];
+
+    const DsStepDev* st = steps + sr.step0;
+    for (int s = threadIdx.x; s < sr.nsteps; s += blockDim.x) {
+        const int64_t kb = ds_first_at(c0, st[s]);
+        const int64_t ke = (c1 >= sr.N) ? st[s].n : ds_first_at(c1, st[s]);
+        s_kbeg[s] = kb;
+        s_cnt_prefix[s + 1] = (int)(ke - kb);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_cnt_prefix[0] = 0;
+        for (int s = 0; s < sr.nsteps; ++s)
+            s_cnt_prefix[s + 1] += s_cnt_prefix[s];
+    }
+    __syncthreads();
+    const int total = s_cnt_prefix[sr.nsteps];
+    for (int e = threadIdx.x; e < total; e += blockDim.x) {
+        int a = 0, z = sr.nsteps - 1;
+        while (a < z) {
+            const int mid = (a + z + 1) >> 1;
+            if (s_cnt_prefix[mid] <= e) a = mid;
+            else z = mid - 1;
+        }
+        const DsStepDev& p = st[a];
+        const int64_t k = s_kbeg[a] + (e - s_cnt_prefix[a]);
+        DsIndex ix = ds_index(k, p.mf, p.ef, sr.N);
+        ix.imin -= c0;
+        ix.imax -= c0;
+        sr.out[p.out_offset + k] = ds_accumulate(s_in, ix);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Running median (running_median.hpp:100-132): exact order statistic of the
+// edge-padded window, found by counting (pure comparisons, bit-exact).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void running_median_kernel(const float* __restrict__ x,
+                                                             int64_t N, int width,
+                                                             int use_lds,
+                                                             float* __restrict__ out)
+{
+    extern __shared__ float s_win[];
+    const int half = width / 2;
+    const int64_t o0 = (int64_t)blockIdx.x * blockDim.x;
+    const int span = blockDim.x + width - 1;
+    if (use_lds) {
+        for (int j = threadIdx.x; j < span; j += blockDim.x) {
+            int64_t src = o0 + j - half;
+            src = src < 0 ? 0 : (src > N - 1 ? N - 1 : src);
+            s_win[j] = x[src];
+        }
+        __syncthreads();
+    }
+    const int64_t i = o0 + threadIdx.x;
+    if (i >= N) return;
+    float med = 0.0f;
+    for (int c = 0; c < width; ++c) {
+        float cand;
+        if (use_lds) {
+            cand = s_win[threadIdx.x + c];
+        } else {
+            int64_t src = i + c - half;
+            src = src < 0 ? 0 : (src > N - 1 ? N - 1 : src);
+            cand = x[src];
+        }
+        int lt = 0, le = 0;
+        for (int j = 0; j < width; ++j) {
+            float v;
+            if (use_lds) {
+                v = s_win[threadIdx.x + j];
+            } else {
+                int64_t src = i + j - half;
+                src = src < 0 ? 0 : (src > N - 1 ? N - 1 : src);
+                v = x[src];
+            }
+            lt += (v < cand);
+            le += (v <= cand);
+        }
+        if (lt <= half && half < le) {
+            med = cand;
+            break;
+        }
+    }
+    out[i] = med;
+}
+
+// ---------------------------------------------------------------------------
+// Dereddening (running_medians.py:40-85, time_series.py:95-124)
+// ---------------------------------------------------------------------------
+
+// numpy's float32 pairwise summation (8 accumulators on blocks of <= 128,
+// halving split otherwise) of a[0..n).
+__device__ __forceinline__ float pairwise_leaf(const float* a, int64_t n)
+{
+    if (n < 8) {
+        float res = 0.0f;
+        for (int64_t i = 0; i < n; ++i) res += a[i];
+        return res;
+    }
+    float r0 = a[0], r1 = a[1], r2 = a[2], r3 = a[3];
+    float r4 = a[4], r5 = a[5], r6 = a[6], r7 = a[7];
+    int64_t i = 8;
+    for (; i < n - (n % 8); i += 8) {
+        r0 += a[i + 0]; r1 += a[i + 1]; r2 += a[i + 2]; r3 += a[i + 3];
+        r4 += a[i + 4]; r5 += a[i + 5]; r6 += a[i + 6]; r7 += a[i + 7];
+    }
+    float res = ((r0 + r1) + (r2 + r3)) + ((r4 + r5) + (r6 + r7));
+    for (; i < n; ++i) res += a[i];
+    return res;
+}
+
+__device__ float pairwise_sum(const float* a, int64_t n)
+{
+    // explicit stack replaces numpy's recursion; combines results in the
+    // same (left + right) order
+    struct Frame {
+        const float* a;
+        int64_t n;
+        float left;
+        int state;
+    };
+    Frame st[40];
+    int sp = 0;
+    st[0].a = a;
+    st[0].n = n;
+    st[0].state = 0;
+    float ret = 0.0f;
+    for (;;) {
+        Frame& f = st[sp];
+        if (f.n <= 128) {
+            ret = pairwise_leaf(f.a, f.n);
+            if (sp == 0) return ret;
+            --sp;
+            continue;  // deliver ret to the parent
+        }
+        int64_t n2 = f.n / 2;
+        n2 -= n2 % 8;
+        if (f.state == 0) {
+            f.state = 1;
+            Frame& c = st[++sp];
+            c.a = f.a;
+            c.n = n2;
+            c.state = 0;
+        } else if (f.state == 1) {
+            f.left = ret;
+            f.state = 2;
+            Frame& c = st[++sp];
+            c.a = f.a + n2;
+            c.n = f.n - n2;
+            c.state = 0;
+        } else {
+            ret = f.left + ret;
+            if (sp == 0) return ret;
+            --sp;
+        }
+    }
+}
+
+// Block means of `factor` samples: numpy float32 mean = (0 + pairwise sum)/factor.
+__global__ void scrunch_mean_kernel(const float* __restrict__ x, int64_t nblocks, int64_t factor,
+                                    float* __restrict__ out)
+{
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblocks;
+         b += (int64_t)gridDim.x * blockDim.x) {
+        const float s = 0.0f + pairwise_sum(x + b * factor, factor);
+        out[b] = s / (float)factor;
+    }
+}
+
+// out[i] = float32(x[i] - interp(i)) (or float32(interp(i)) when x is null),
+// where interp is numpy's linear
+// interpolation of the low-resolution running median at centres
+// j*sf + (sf-1)/2 (np.interp semantics: clamp to the end values, exact node
+// values on the nodes).  Evaluated in float-float arithmetic, rounded once.
+__global__ void deredden_subtract_kernel(const float* __restrict__ x, int64_t N,
+                                         const float* __restrict__ fp, int64_t nl, int64_t sf,
+                                         float* __restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        // 2*x_j = 2*j*sf + sf - 1; node j is the last with 2*x_j <= 2*i
+        const int64_t num = 2 * i - sf + 1;
+        int64_t j = (num >= 0) ? num / (2 * sf) : -1;
+        float vh, vl;
+        if (j < 0) {
+            vh = fp[0];
+            vl = 0.0f;
+        } else if (j >= nl - 1) {
+            vh = fp[nl - 1];
+            vl = 0.0f;
+        } else {
+            const int64_t t2 = num - 2 * j * sf;  // 2*(i - x_j), in [0, 2*sf)
+            if (t2 == 0) {
+                vh = fp[j];
+                vl = 0.0f;
+            } else {
+                // (fp[j+1] - fp[j]) * (t2/2) / sf + fp[j]
+                float dh, dl;
+                two_sum(fp[j + 1], -fp[j], dh, dl);
+                const float t = 0.5f * (float)t2;  // exact (t2 < 2^24)
+                const float ph = dh * t;
+                const float pl = __fmaf_rn(dh, t, -ph) + dl * t;
+                const float fsf = (float)sf;
+                const float qh = ph / fsf;
+                const float r = __fmaf_rn(-qh, fsf, ph) + pl;
+                const float ql = r / fsf;
+                float sh, se;
+                two_sum(fp[j], qh, sh, se);
+                vh = sh;
+                vl = se + ql;
+            }
+        }
+        if (x == nullptr) {
+            out[i] = vh + vl;  // the interpolated running median alone
+            continue;
+        }
+        float d, e;
+        two_sum(x[i], -vh, d, e);
+        e -= vl;
+        out[i] = d + e;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Normalisation (time_series.py:67-93): float-float sums, fixed reduction
+// order, one rounding of (x - mean)/std.
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ void block_ff_reduce(float& h, float& l, float* s_h, float* s_l)
+{
+    // wavefront reduce then across wavefronts, fixed order
+    for (int d = 32; d >= 1; d >>= 1) {
+        const float oh = __shfl_down(h, d, kWave);
+        const float ol = __shfl_down(l, d, kWave);
+        ff_add(h, l, oh, ol, h, l);
+    }
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    if (lane == 0) {
+        s_h[wave] = h;
+        s_l[wave] = l;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float rh = s_h[0], rl = s_l[0];
+        for (int w = 1; w < (int)(blockDim.x / kWave); ++w)
+            ff_add(rh, rl, s_h[w], s_l[w], rh, rl);
+        h = rh;
+        l = rl;
+    }
+}
+
+// mode 0: partial sums of x; mode 1: partial sums of (x - mean)^2
+__global__ __launch_bounds__(256) void moments_partial_kernel(const float* __restrict__ x,
+                                                              int64_t N, int mode,
+                                                              const float* __restrict__ mean,
+                                                              float* __restrict__ partial)
+{
+    __shared__ float s_h[4], s_l[4];
+    float h = 0.0f, l = 0.0f;
+    const float mh = mode ? mean[0] : 0.0f;
+    const float ml = mode ? mean[1] : 0.0f;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = x[i];
+        if (mode == 0) {
+            ff_add(h, l, v, 0.0f, h, l);
+        } else {
+            float dh, dl;
+            two_sum(v, -mh, dh, dl);
+            dl -= ml;
+            const float sh = dh * dh;
+            const float sl = __fmaf_rn(dh, dh, -sh) + 2.0f * dh * dl;
+            ff_add(h, l, sh, sl, h, l);
+        }
+    }
+    block_ff_reduce(h, l, s_h, s_l);
+    if (threadIdx.x == 0) {
+        partial[2 * blockIdx.x] = h;
+        partial[2 * blockIdx.x + 1] = l;
+    }
+}
+
+// Reduces the partials; mode 0 writes mean = sum/N, mode 1 writes
+// std = sqrt(sum/N), both as float-float pairs.
+__global__ __launch_bounds__(256) void moments_final_kernel(const float* __restrict__ partial,
+                                                            int nparts, int64_t N, int mode,
+                                                            float* __restrict__ result)
+{
+    __shared__ float s_h[4], s_l[4];
+    float h = 0.0f, l = 0.0f;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x)
+        ff_add(h, l, partial[2 * i], partial[2 * i + 1], h, l);
+    block_ff_reduce(h, l, s_h, s_l);
+    if (threadIdx.x == 0) {
+        // divide the float-float sum by N (exact in float for N < 2^24 only:
+        // use a float-float divisor)
+        const float nh = (float)N;
+        const float nl = (float)(N - (int64_t)nh);
+        float qh = h / nh;
+        // remainder r = (h + l) - qh*(nh + nl)
+        const float p = qh * nh;
+        const float pe = __fmaf_rn(qh, nh, -p);
+        float r = ((h - p) - pe) + l - qh * nl;
+        float ql = r / nh;
+        float mh, ml;
+        mh = qh + ql;
+        ml = ql - (mh - qh);
+        if (mode == 1) {
+            // sqrt of a float-float: one Newton step from the float root
+            const float s = sqrtf(mh);
+            const float ss = s * s;
+            const float sse = __fmaf_rn(s, s, -ss);
+            const float corr = ((mh - ss) - sse + ml) / (2.0f * s);
+            mh = s + corr;
+            ml = corr - (mh - s);
+        }
+        result[0] = mh;
+        result[1] = ml;
+    }
+}
+
+// out = float32((x - mean) / std) with float-float mean and std
+__global__ void normalise_apply_kernel(const float* __restrict__ x, int64_t N,
+                                       const float* __restrict__ mean,
+                                       const float* __restrict__ stdv, float* __restrict__ out)
+{
+    const float mh = mean[0], ml = mean[1];
+    const float sh = stdv[0], sl = stdv[1];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float dh, dl;
+        two_sum(x[i], -mh, dh, dl);
+        dl -= ml;
+        // (dh + dl) / (sh + sl)
+        const float q = dh / sh;
+        const float p = q * sh;
+        const float pe = __fmaf_rn(q, sh, -p);
+        const float r = ((dh - p) - pe) + dl - q * sl;
+        out[i] = q + r / sh;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Binding helpers (kernels.hpp:19-38, 73-101)
+// ---------------------------------------------------------------------------
+
+__global__ void rollback_kernel(const float* __restrict__ x, int64_t size, int64_t shift,
+                                float* __restrict__ out)
+{
+    const int64_t p = shift % size;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t j = i + p;
+        if (j >= size) j -= size;
+        out[i] = x[j];
+    }
+}
+
+__global__ void fused_rollback_add_kernel(const float* __restrict__ x,
+                                          const float* __restrict__ y, int64_t size,
+                                          int64_t shift, float* __restrict__ z)
+{
+    const int64_t p = shift % size;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < size;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t j = i + p;
+        if (j >= size) j -= size;
+        z[i] = x[i] + y[j];
+    }
+}
+
+// Circular prefix sum over nsum positions: float-float scan of the first
+// wrap (rounded to float32), then the float32 wraps out[j] + i*sum.
+__global__ __launch_bounds__(64) void circular_prefix_sum_kernel(const float* __restrict__ x,
+                                                                 int64_t size, int64_t nsum,
+                                                                 float* __restrict__ out)
+{
+    const int lane = threadIdx.x;
+    const int64_t first = size < nsum ? size : nsum;
+    snr_prefix_long(x, out, (int)first, lane);
+    __syncthreads();
+    if (nsum <= size) return;
+    const float total = out[size - 1];
+    const int64_t wraps = nsum / size;
+    const int64_t rest = nsum % size;
+    for (int64_t i = 1; i < wraps; ++i) {
+        const float a = (float)i * total;
+        for (int64_t j = lane; j < size; j += kWave)
+            out[i * size + j] = out[j] + a;
+    }
+    const float a = (float)wraps * total;
+    for (int64_t j = lane; j < rest; j += kWave)
+        out[wraps * size + j] = out[j] + a;
+}
+
+__global__ void fill_zero_kernel(float* __restrict__ p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0.0f;
+}
+
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+void split_double(double f, uint64_t* mf, int* ef)
+{
+    int e = 0;
+    const double m = frexp(f, &e);  // f = m * 2^e, 0.5 <= m < 1
+    *mf = (uint64_t)ldexp(m, 53);
+    *ef = e - 53;
+}
+
+static inline int grid_for(int64_t n, int threads, int cap = 65536)
+{
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+
+hipError_t launch_downsample_direct(const float* in, int64_t N, double f, int64_t n, float* out,
+                                    hipStream_t stream)
+{
+    uint64_t mf;
+    int ef;
+    split_double(f, &mf, &ef);
+    hipLaunchKernelGGL(downsample_direct_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, in,
+                       N, mf, ef, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_downsample_packed(const DsSeriesDev* d_series, int nseries,
+                                    const DsStepDev* d_steps, int total_chunks, int chunk,
+                                    int halo, hipStream_t stream)
+{
+    const size_t lds = (size_t)(chunk + halo) * sizeof(float);
+    hipLaunchKernelGGL(downsample_packed_kernel, dim3(total_chunks), dim3(kDsThreads), lds, stream,
+                       d_series, nseries, d_steps, chunk, halo);
+    return hipGetLastError();
+}
+
+hipError_t launch_running_median(const float* x, int64_t N, int width, float* out,
+                                 hipStream_t stream)
+{
+    const int threads = 256;
+    const size_t span = (size_t)threads + width - 1;
+    const int use_lds = span * sizeof(float) <= 48 * 1024;
+    hipLaunchKernelGGL(running_median_kernel, dim3((unsigned)((N + threads - 1) / threads)),
+                       dim3(threads), use_lds ? span * sizeof(float) : 0, stream, x, N, width,
+                       use_lds, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_scrunch_mean(const float* x, int64_t nblocks, int64_t factor, float* out,
+                               hipStream_t stream)
+{
+    hipLaunchKernelGGL(scrunch_mean_kernel, dim3(grid_for(nblocks, 64)), dim3(64), 0, stream, x,
+                       nblocks, factor, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_deredden_subtract(const float* x, int64_t N, const float* fp, int64_t nl,
+                                    int64_t sf, float* out, hipStream_t stream)
+{
+    hipLaunchKernelGGL(deredden_subtract_kernel, dim3(grid_for(N, 256)), dim3(256), 0, stream, x,
+                       N, fp, nl, sf, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalise(const float* x, int64_t N, float* out, float* work, int parts,
+                            hipStream_t stream)
+{
+    float* partial = work;
+    float* mean = work + 2 * parts;
+    float* stdv = mean + 2;
+    hipLaunchKernelGGL(moments_partial_kernel, dim3(parts), dim3(256), 0, stream, x, N, 0,
+                       (const float*)nullptr, partial);
+    hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(256), 0, stream, partial, parts, N, 0,
+                       mean);
+    hipLaunchKernelGGL(moments_partial_kernel, dim3(parts), dim3(256), 0, stream, x, N, 1,
+                       (const float*)mean, partial);
+    hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(256), 0, stream, partial, parts, N, 1,
+                       stdv);
+    hipLaunchKernelGGL(normalise_apply_kernel, dim3(grid_for(N, 256)), dim3(256), 0, stream, x, N,
+                       (const float*)mean, (const float*)stdv, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_rollback(const float* x, int64_t size, int64_t shift, float* out,
+                           hipStream_t stream)
+{
+    hipLaunchKernelGGL(rollback_kernel, dim3(grid_for(size, 256)), dim3(256), 0, stream, x, size,
+                       shift, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_rollback_add(const float* x, const float* y, int64_t size, int64_t shift,
+                                     float* z, hipStream_t stream)
+{
+    hipLaunchKernelGGL(fused_rollback_add_kernel, dim3(grid_for(size, 256)), dim3(256), 0, stream,
+                       x, y, size, shift, z);
+    return hipGetLastError();
+}
+
+hipError_t launch_circular_prefix_sum(const float* x, int64_t size, int64_t nsum, float* out,
+                                      hipStream_t stream)
+{
+    hipLaunchKernelGGL(circular_prefix_sum_kernel, dim3(1), dim3(64), 0, stream, x, size, nsum,
+                       out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t stream)
+{
+    hipLaunchKernelGGL(fill_zero_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace rt

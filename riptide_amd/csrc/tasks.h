@@ -1,0 +1,56 @@
+// Work descriptors shared by the host planner and the gfx950 kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace rt {
+
+// Threads per workgroup of the FFA kernels (16 wavefronts of 64).
+constexpr int kFfaThreads = 1024;
+// Wavefront width on CDNA.
+constexpr int kWave = 64;
+// Largest merge-level count of a cone task (2^kMaxConeLevels bottom segments).
+constexpr int kMaxConeLevels = 7;
+// Register-resident S/N positions per lane of the standalone S/N kernel
+// (rows up to 64*kSnrRegsMaxK bins; longer rows take the chunked path).
+constexpr int kSnrRegsMaxK = 16;
+// Largest number of trial widths whose results one wavefront keeps in
+// registers before flushing (one per lane).
+constexpr int kMaxWidthsPerFlush = 64;
+
+// One FFA work item: a tile of rows [lo, hi) of one node of a transform's
+// merge tree, computed `levels` merge levels up from a source state.
+//
+// The merge tree of an m-row transform splits every node of M rows into a
+// head of M>>1 rows and a tail of M - (M>>1) rows (block.hpp:30-43); a node's
+// transform occupies its own row range.  "State at depth d" is the buffer in
+// which every node at depth d (and every single-row leaf above it) holds its
+// transform.  Depth-0 state is the transform itself; the folded input is the
+// state at the leaf depth.
+struct FfaTask {
+    const float* src;   // source state rows (row r at src + r*bins)
+    float* dst;         // kind 0: destination state rows
+    float* snr;         // kind 1: S/N of transform row s at snr + s*nw
+    const int32_t* widths;  // trial widths (nw)
+    const float* hb;        // per width: (h + b, b) pairs, 2*nw floats
+    int32_t bins;
+    int32_t node_r0;    // node at the output depth: first row
+    int32_t node_m;     // node at the output depth: number of rows
+    int32_t lo, hi;     // tile, node-relative rows [lo, hi)
+    int32_t levels;     // merge levels computed by this task
+    int32_t cone;       // 0: whole node from the folded input; 1: cone tile
+    int32_t kind;       // 0: write the tile rows to dst; 1: boxcar S/N
+    int32_t rows_eval;  // kind 1: only rows s < rows_eval get S/N
+    int32_t nw;
+    float stdnoise;
+    int32_t pad;
+};
+
+// One downsampling step of one series for the packed multi-step kernel.
+struct DsStep {
+    double f;           // downsampling factor (> 1)
+    int64_t n;          // output samples
+    int64_t out_offset; // offset into the packed output
+};
+
+}  // namespace rt

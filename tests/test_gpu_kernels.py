@@ -1,0 +1,94 @@
+"""GPU parity of the kernel-level API against the CPU oracle (same seeded
+inputs): ffa2 and the running median bit-exact; S/N and downsampling within
+the float32 tolerances stated per test."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def lc():
+    from riptide_amd import libcpp
+    return libcpp
+
+
+FFA_SHAPES = [
+    (1, 7), (2, 9), (3, 5), (8, 8), (7, 240), (37, 260), (257, 5), (536, 250),
+    (150, 260),     # one workgroup, whole transform
+    (2684, 260),    # leaf pass + one cone pass
+    (1342, 520),    # leaf pass + two cone passes
+    (5368, 240),    # deepest tree of the search configurations
+    (8195, 5),      # first row count where the float32 row map != integer rounding
+    (40, 3000),     # rows too wide to tile: global-memory levels
+    (5, 20000),
+]
+
+
+@pytest.mark.parametrize("shape", FFA_SHAPES)
+def test_ffa2_bit_exact(lc, oracle, shape):
+    rng = np.random.default_rng(hash(shape) % 2**32)
+    x = rng.standard_normal(shape).astype(np.float32)
+    got = lc.ffa2(x)
+    want = oracle.oracle().ffa2(x)
+    assert got.dtype == np.float32 and got.shape == x.shape
+    assert np.array_equal(got, want), f"max |diff| {np.abs(got - want).max()}"
+
+
+def test_ffa2_kat_8x8(lc):
+    x = np.zeros((8, 8), np.float32)
+    x[:, 7] = 1
+    out = lc.ffa2(x)
+    assert np.array_equal(out[0], [0, 0, 0, 0, 0, 0, 0, 8])
+    assert np.array_equal(out[7], [1] * 8)
+
+
+@pytest.mark.parametrize("rows,cols", [(1, 64), (50, 260), (300, 240), (7, 520), (3, 1500)])
+def test_snr2_parity(lc, oracle, rows, cols):
+    rng = np.random.default_rng(rows * 1000 + cols)
+    x = (rng.standard_normal((rows, cols)) * 5).astype(np.float32)
+    w = oracle.generate_width_trials(min(cols - 1, 480))
+    w = w[w < cols]
+    got = lc.snr2(x, w, 1.7)
+    want = oracle.oracle().snr2(x, w, 1.7)
+    # S/N tolerance of the search: 1e-5 relative and 1e-4 absolute
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-4)
+    # the float-float scan reproduces the float64 prefix sums: nearly all bits agree
+    assert np.mean(got == want) > 0.99
+
+
+@pytest.mark.parametrize("f", [1.5, 2.0, 6.510416666666667, 13.777, 600.3, 4999.5])
+def test_downsample_parity(lc, oracle, f):
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(200003).astype(np.float32)
+    got = lc.downsample(x, f)
+    want = oracle.oracle().downsample(x, f)
+    assert got.shape == want.shape
+    # same index bounds and edge weights; only the float32 summation order
+    # differs from the vectorised reference loop
+    np.testing.assert_allclose(got, want, rtol=0, atol=4e-7 * np.sqrt(f) * 8)
+
+
+@pytest.mark.parametrize("width", [1, 3, 5, 7, 11, 25, 37, 101])
+def test_running_median_bit_exact(lc, oracle, width):
+    rng = np.random.default_rng(width)
+    x = rng.standard_normal(5000).astype(np.float32)
+    assert np.array_equal(lc.running_median(x, width), oracle.oracle().running_median(x, width))
+
+
+def test_helpers(lc, oracle):
+    rng = np.random.default_rng(3)
+    x = rng.standard_normal(37).astype(np.float32)
+    y = rng.standard_normal(37).astype(np.float32)
+    for shift in (0, 1, 5, 36, 37, 100):
+        assert np.array_equal(lc.rollback(x, shift), np.roll(x, -shift))
+        assert np.array_equal(lc.fused_rollback_add(x, y, shift), x + np.roll(y, -shift))
+    for nsum in (10, 37, 50, 111):
+        got = lc.circular_prefix_sum(x, nsum)
+        want = oracle.oracle().circular_prefix_sum(x, nsum)
+        np.testing.assert_allclose(got, want, rtol=0, atol=2e-6)
+
+
+def test_benchmark_ffa2(lc):
+    t = lc.benchmark_ffa2(64, 256, 3)
+    assert 0 < t < 1.0

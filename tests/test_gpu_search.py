@@ -1,0 +1,136 @@
+"""GPU parity of the search path (dereddening, normalisation, periodogram,
+ffa_search) against the CPU oracle on the same seeded inputs.
+
+Tolerances: trial periods and fold bins bit-exact; S/N within 1e-5 relative
+and 1e-4 absolute of the oracle; dereddened / normalised series within a few
+float32 ulps (the reference rounds float64 numpy results to float32, the GPU
+rounds float-float results once)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SNR_RTOL, SNR_ATOL = 1e-5, 1e-4
+
+
+def _pulsar(n, tsamp, period, ducy, amplitude, seed):
+    rng = np.random.default_rng(seed)
+    t = np.arange(n) * tsamp
+    phase = (t / period) % 1.0
+    sig = np.exp(-0.5 * ((phase - 0.5) / (ducy / 2.355)) ** 2)
+    sig *= amplitude / np.sqrt((sig ** 2).sum())
+    return (sig + rng.standard_normal(n)).astype(np.float32)
+
+
+PGRAM_CASES = [
+    # N, tsamp, pmin, pmax, bmin, bmax
+    (2 ** 17, 256e-6, 0.5, 2.0, 240, 260),
+    (2 ** 17, 256e-6, 0.5, 2.0, 480, 520),
+    (200_000, 1e-3, 0.8, 1.2, 240, 260),
+    (200_000, 1e-3, 0.8, 1.2, 800, 1200),   # period_min = bins_min * tsamp: first step f == 1
+    (2 ** 20, 64e-6, 0.1, 1.0, 480, 520),   # multi-pass transforms (up to ~1300 rows)
+]
+
+
+@pytest.mark.parametrize("case", PGRAM_CASES)
+def test_periodogram_parity(oracle, case):
+    from riptide_amd import libcpp
+    N, tsamp, pmin, pmax, bmin, bmax = case
+    x = _pulsar(N, tsamp, 0.3711 * (pmin + pmax), 0.03, 15.0, seed=N % 1000)
+    x = oracle.normalise(x)
+    w = oracle.generate_width_trials(bmin)
+    P, F, S = libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    assert P.shape == Po.shape and S.shape == So.shape
+    assert np.array_equal(P, Po)
+    assert np.array_equal(F, Fo)
+    np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)
+
+
+def test_plain_schedule_matches(oracle):
+    from riptide_amd import _lib, libcpp
+    N, tsamp, pmin, pmax, bmin, bmax = 2 ** 18, 128e-6, 0.2, 2.0, 240, 260
+    x = oracle.normalise(_pulsar(N, tsamp, 0.77, 0.02, 20.0, seed=5))
+    w = oracle.generate_width_trials(bmin)
+    d = _lib.to_device(x)
+    a = libcpp.periodogram_device(d, N, tsamp, w, pmin, pmax, bmin, bmax, plain=False)
+    b = libcpp.periodogram_device(d, N, tsamp, w, pmin, pmax, bmin, bmax, plain=True)
+    for u, v in zip(a, b):
+        assert np.array_equal(_lib.to_numpy(u), _lib.to_numpy(v))
+
+
+def _ulps(a, b):
+    a = np.asarray(a, np.float32)
+    b = np.asarray(b, np.float32)
+    return np.abs(a.astype(np.float64) - b) / np.spacing(np.maximum(np.abs(a), np.abs(b)))
+
+
+@pytest.mark.parametrize("n,tsamp,width,minpts", [
+    (2 ** 18, 256e-6, 4.0, 101),   # scrunch factor 154
+    (100_000, 1e-3, 0.5, 51),      # scrunch factor 9
+    (3000, 1e-3, 0.05, 101),       # no scrunching: exact running median
+])
+def test_deredden_parity(oracle, n, tsamp, width, minpts):
+    from riptide_amd.time_series import deredden_array
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 3 * np.sin(np.arange(n) / 5000.0)).astype(np.float32)
+    got = deredden_array(x, int(round(width / tsamp)), minpts)
+    want = oracle.deredden(x, tsamp, width, minpts)
+    # rounding of a float64 (reference) vs float-float (GPU) difference
+    scale = np.abs(x).max()
+    np.testing.assert_allclose(got, want, rtol=0, atol=2 * np.spacing(np.float32(scale)))
+    assert np.mean(got == want) > 0.99
+
+
+def test_normalise_parity(oracle):
+    from riptide_amd.time_series import normalise_array
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal(1_000_003) * 3 + 42).astype(np.float32)
+    got = normalise_array(x)
+    want = oracle.normalise(x)
+    assert np.max(_ulps(got, want)) <= 1
+    assert np.mean(got == want) > 0.99
+
+
+def test_ffa_search_parity(oracle):
+    import riptide_amd as rt
+    N, tsamp = 2 ** 18, 256e-6
+    x = _pulsar(N, tsamp, 1.3711, 0.02, 20.0, seed=1)
+    ts = rt.TimeSeries(x, tsamp)
+    tsdr, pg = rt.ffa_search(ts, period_min=0.5, period_max=2.0, bins_min=240, bins_max=260,
+                             rmed_width=4.0)
+    xo, wo, Po, Fo, So = oracle.ffa_search(x, tsamp, period_min=0.5, period_max=2.0,
+                                           bins_min=240, bins_max=260, rmed_width=4.0)
+    assert np.array_equal(pg.widths, wo)
+    assert np.array_equal(pg.periods, Po)
+    assert np.array_equal(pg.foldbins, Fo)
+    np.testing.assert_allclose(tsdr.data, xo, rtol=0, atol=4e-6)
+    np.testing.assert_allclose(pg.snrs, So, rtol=SNR_RTOL, atol=SNR_ATOL)
+    assert pg.metadata == ts.metadata == tsdr.metadata
+    assert pg.tobs == ts.length
+
+
+def test_rseek_end_to_end_pin():
+    """tests/test_rseek.py:33-62 of the reference: seed-0 fake pulsar, P = 1 s,
+    tobs 128 s, tsamp 256 us, amplitude 20, duty cycle 2%, searched over
+    0.5-2 s with 480-520 bins: best S/N 18.5 +/- 0.15 at width 13 within
+    0.1/tobs of 1 Hz."""
+    import riptide_amd as rt
+    np.random.seed(0)
+    ts = rt.TimeSeries.generate(128.0, 256e-6, 1.0, amplitude=20.0, ducy=0.02, stdnoise=1.0)
+    _, pg = rt.ffa_search(ts, period_min=0.5, period_max=2.0, bins_min=480, bins_max=520)
+    assert pg.snrs.shape == (95033, 12)
+    sel = np.abs(pg.freqs - 1.0) < 0.1 / 128.0
+    best = pg.snrs[sel].max(axis=0)
+    assert abs(best.max() - 18.5) < 0.15
+    assert pg.widths[best.argmax()] == 13
+
+
+def test_ffa_search_returns_same_object():
+    import riptide_amd as rt
+    np.random.seed(1)
+    ts = rt.TimeSeries.generate(20.0, 1e-3, 1.0, amplitude=20.0)
+    tsdr, pg = rt.ffa_search(ts, period_min=0.8, period_max=1.2, bins_min=240, bins_max=260,
+                             already_normalised=True, deredden=False)
+    assert id(tsdr) == id(ts)
+    assert all(np.maximum.accumulate(pg.periods) == pg.periods)

@@ -1,0 +1,273 @@
+"""Benchmark: batched ffa_search over DM trials on one MI355X.
+
+Workload (one "step"): ffa_search -- dereddening (4 s running median),
+normalisation, periodogram over 0.1-10 s with bins (240, 260) or (480, 520)
+picked per series by a seeded coin -- on a batch of 512 float32 series of 2^22
+samples at 64 us (white + red noise, 1 series in 16 with a pulse), inputs
+resident in HBM, S/N outputs left in HBM.  value = DM trials per second.
+
+Also reported on the same line: the plain one-series-at-a-time schedule
+(per-step launches, no queue, no graph), per-phase HIP-event timings, the HBM
+roofline of the dominant kernel (persistent FFA + S/N kernel), and the CPU
+oracle timed on the host cores on a bounded sample of the same workload.
+
+  python bench.py [--steps K] [--warmup W] [--trials T] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+LDS_PEAK_GBS = 78_600.0        # 128 B/clk/CU x 256 CU x 2.4 GHz (nominal)
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(torch, n_series, nsamp, tsamp, seed, dev):
+    """White + red noise (PSD ~ f^-1.5, equal to the white level at 0.1 Hz),
+    one series in 16 with a Gaussian pulse train (P log-uniform 0.2-8 s, duty
+    1-5 %, S/N 8-30); generated on the GPU."""
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    rs = np.random.RandomState(seed)
+    f = torch.fft.rfftfreq(nsamp, d=tsamp, device=dev)
+    shape = torch.zeros_like(f)
+    shape[1:] = (f[1:] / 0.1) ** (-0.75)
+    t = torch.arange(nsamp, device=dev, dtype=torch.float64) * tsamp
+    xs = []
+    for i in range(n_series):
+        x = torch.randn(nsamp, generator=g, device=dev)
+        w = torch.fft.rfft(torch.randn(nsamp, generator=g, device=dev))
+        x += torch.fft.irfft(w * shape, n=nsamp)
+        if i % 16 == 0:
+            period = float(np.exp(rs.uniform(np.log(0.2), np.log(8.0))))
+            duty = rs.uniform(0.01, 0.05)
+            snr = rs.uniform(8, 30)
+            ph = torch.remainder(t / period, 1.0)
+            sig = torch.exp(-0.5 * ((ph - 0.5) / (duty / 2.355)) ** 2)
+            sig *= snr / torch.sqrt((sig ** 2).sum())
+            x += sig.float()
+        xs.append(x.contiguous())
+    return xs
+
+
+def _cpu_worker(args):
+    """One host core: the oracle ffa_search on one trial of each bins class."""
+    nsamp, tsamp, seed = args
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    x = np.random.default_rng(seed).standard_normal(nsamp).astype(np.float32)
+    times = []
+    for b0, b1 in ((240, 260), (480, 520)):
+        t0 = time.perf_counter()
+        O.ffa_search(x, tsamp, period_min=0.1, period_max=10.0, bins_min=b0, bins_max=b1)
+        times.append(time.perf_counter() - t0)
+    return times
+
+
+def cpu_baseline(nsamp, tsamp):
+    import multiprocessing as mp
+    cores = max(1, min(16, os.cpu_count() or 1))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(nsamp, tsamp, 100 + c) for c in range(cores)])
+    wall = max(sum(r) for r in res)
+    per = np.mean(np.asarray(res), axis=0)
+    return {
+        "value": round(2 * cores / wall, 4),
+        "unit": "DM trials/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle ffa_search (C restatement of the reference kernels + its numpy "
+                   f"dereddening/normalisation), one 2^22-sample trial of each bins class per "
+                   f"core, {cores} single-threaded processes; per-core s/trial: "
+                   f"bins 240-260 {per[0]:.2f}, bins 480-520 {per[1]:.2f}"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--trials", type=int, default=512)
+    ap.add_argument("--nsamp", type=int, default=2 ** 22)
+    ap.add_argument("--plain-trials", type=int, default=16)
+    ap.add_argument("--group-size", type=int, default=64)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--seed", type=int, default=20260)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); DM trials are "
+              "independent, so a multi-GPU run shards trials across GPUs with no exchange.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from riptide_amd import _lib, libcpp
+    from riptide_amd.batch import BatchSearch
+    from riptide_amd.ffautils import generate_width_trials
+
+    dev = _lib.device()
+    stream = torch.cuda.Stream()
+    tsamp = 64e-6
+    pmin, pmax = 0.1, 10.0
+    coin = np.random.RandomState(args.seed).randint(0, 2, size=args.trials)
+    bmin = np.where(coin == 0, 240, 480)
+    bmax = np.where(coin == 0, 260, 520)
+
+    with torch.cuda.stream(stream):
+        t0 = time.time()
+        xs = make_inputs(torch, args.trials, args.nsamp, tsamp, args.seed, dev)
+        torch.cuda.synchronize()
+        log(f"inputs: {args.trials} x {args.nsamp} samples in {time.time() - t0:.1f} s")
+
+        batch = BatchSearch([args.nsamp] * args.trials, tsamp, bmin, bmax, pmin, pmax,
+                            deredden=True, rmed_width=4.0, already_normalised=False,
+                            group_size=args.group_size)
+        outs = batch.allocate_outputs()
+        batch.bind(xs, outs)
+        st = batch.stats()
+        log(f"batch: {st}")
+        use_graph = not args.no_graph
+
+        for _ in range(args.warmup):
+            batch.run(use_graph=use_graph)
+        batch.check()
+        torch.cuda.synchronize()
+
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(args.steps):
+            batch.run(use_graph=use_graph)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        batched_ms = ev0.elapsed_time(ev1) / args.steps
+        batch.check()
+        trials_per_s = args.trials / (batched_ms * 1e-3)
+        log(f"batched: {batched_ms:.1f} ms per batch -> {trials_per_s:.1f} trials/s")
+
+        # per-phase event timing of one eager run (same kernels, same stream)
+        phases = batch.run_timed()
+        log(f"phases: {phases}")
+
+        # parity spot check of series 0 against the CPU oracle
+        from oracle import oracle as O
+        x0 = xs[0].cpu().numpy()
+        _, _, Po, Fo, So = O.ffa_search(x0, tsamp, period_min=pmin, period_max=pmax,
+                                        bins_min=int(bmin[0]), bins_max=int(bmax[0]))
+        S0 = outs[0].cpu().numpy()
+        P0, F0 = batch.grid(0)
+        parity = {
+            "series": 0,
+            "periods_bit_exact": bool(np.array_equal(P0, Po) and np.array_equal(F0, Fo)),
+            "snr_max_abs_diff": float(np.max(np.abs(S0 - So))),
+            "snr_within_1e-5rel_1e-4abs": bool(np.allclose(S0, So, rtol=1e-5, atol=1e-4)),
+        }
+        log(f"parity: {parity}")
+
+        # plain schedule: one series at a time, per-step launches, no graph
+        nplain = min(args.plain_trials, args.trials)
+        idx = list(range(nplain))
+        widths = {b: generate_width_trials(b) for b in (240, 480)}
+        tmp = _lib.empty((args.nsamp,))
+        tmp2 = _lib.empty((args.nsamp,))
+        plain_out = {b: libcpp.periodogram_device(tmp, args.nsamp, tsamp, widths[b], pmin, pmax,
+                                                  b, b + (20 if b == 240 else 40))
+                     for b in (240, 480)}  # warm-up + buffers
+        torch.cuda.synchronize()
+        ws = int(round(4.0 / tsamp))
+        t0 = time.perf_counter()
+        for i in idx:
+            b0, b1 = int(bmin[i]), int(bmax[i])
+            _lib.call("rt_deredden", _lib.ptr(xs[i]), args.nsamp, ws, 101, _lib.ptr(tmp),
+                      _lib.stream())
+            _lib.call("rt_normalise", _lib.ptr(tmp), args.nsamp, _lib.ptr(tmp2), _lib.stream())
+            P, F, S = plain_out[b0]
+            w = np.ascontiguousarray(widths[b0], dtype=np.uint64)
+            _lib.call("rt_periodogram_plain", _lib.ptr(tmp2), args.nsamp, tsamp,
+                      _lib.host_ptr(w), w.size, pmin, pmax, b0, b1, _lib.ptr(P), _lib.ptr(F),
+                      _lib.ptr(S), _lib.stream())
+        torch.cuda.synchronize()
+        plain_s = time.perf_counter() - t0
+        plain_tps = nplain / plain_s
+        log(f"plain: {nplain} trials in {plain_s:.3f} s -> {plain_tps:.1f} trials/s")
+
+    # roofline of the dominant kernel: persistent FFA + S/N kernel
+    snr_bytes = sum(batch.lengths[i] * batch.widths[i].size * 4 for i in range(args.trials))
+    fold_bytes = st["fold_floats"] * 4
+    launches = max(1, phases["ffa_launches"])
+    bytes_per_launch = (fold_bytes + snr_bytes) / launches
+    ms_per_launch = phases["ffa_ms"] / launches
+    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
+    lds_bytes = 8 * st["ffa_adds"]
+    lds_achieved = lds_bytes / (phases["ffa_ms"] * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        t0 = time.time()
+        cpu = cpu_baseline(args.nsamp, tsamp)
+        log(f"cpu baseline: {cpu} ({time.time() - t0:.1f} s)")
+
+    line = {
+        "metric": "DM trials/sec (2^22-sample ffa_search), batched vs plain, 1 GPU; fwd+bwd ms",
+        "value": round(trials_per_s, 2),
+        "unit": "DM trials/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batched_ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (white + red noise, 1 in 16 with a pulse), generated on the GPU",
+        "config": {
+            "workload": (f"batched ffa_search: {args.trials} DM trials x {args.nsamp} samples, "
+                         f"tsamp 64 us, P 0.1-10 s, bins (240,260)/(480,520) by seeded coin, "
+                         f"dereddening 4 s + normalisation, S/N left in HBM"),
+            "trials": args.trials,
+            "nsamp": args.nsamp,
+            "bins_classes": {"240-260": int((coin == 0).sum()), "480-520": int((coin == 1).sum())},
+            "hipgraph": use_graph,
+            "group_size": args.group_size,
+        },
+        "plain": {"value": round(plain_tps, 2), "unit": "DM trials/s", "trials": nplain,
+                  "schedule": "one series per launch sequence: per-step launches, no queue, no graph"},
+        "batched_over_plain": round(trials_per_s / plain_tps, 3),
+        "phases_ms": {k: round(v, 3) if isinstance(v, float) else v for k, v in phases.items()},
+        "parity": parity,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "ffa_batch_kernel (fold + FFA merge levels + boxcar S/N)",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": None,
+            "algorithmic_bytes_per_launch": int(bytes_per_launch),
+            "ms_per_launch": round(ms_per_launch, 3),
+            "lds": {"achieved": round(lds_achieved, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(lds_achieved / LDS_PEAK_GBS, 4),
+                    "bytes": "8 B per FFA float add"},
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -123,6 +123,48 @@ int rt_fast_running_median(const float* d_in, int64_t size, int64_t width_sample
 /* TimeSeries.normalise (time_series.py:67-93): zero mean, unit variance. */
 int rt_normalise(const float* d_in, int64_t size, float* d_out, void* stream);
 
+/* ---- Batched DM-trial search (no counterpart in the reference: it replaces
+ * the reference's one-process-per-DM-trial pool, pipeline/worker_pool.py:35-45,
+ * each worker calling ffa_search, search.py:10-93) -------------------------- */
+
+/* Plans a batch of nseries searches.  Series i has sizes[i] samples, sampling
+ * time tsamps[i], bins range [bins_min[i], bins_max[i]] and widths_count[i]
+ * trial widths (consecutive in widths_flat).  deredden / normalise select the
+ * preparation of ffa_search (search.py:82-86).  Series are processed in
+ * groups of group_size.  Allocates every device buffer the batch needs. */
+int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
+                    const uint64_t* widths_flat, const int64_t* widths_count, double period_min,
+                    double period_max, const int64_t* bins_min, const int64_t* bins_max,
+                    int deredden, double rmed_width, int64_t rmed_minpts, int normalise,
+                    int64_t group_size, void** handle);
+
+/* Number of trial periods of series i, and its period / fold-bins grid. */
+int rt_batch_length(void* handle, int64_t i, int64_t* length);
+int rt_batch_grid(void* handle, int64_t i, double* h_periods, uint32_t* h_foldbins);
+
+/* Binds device input series and S/N outputs (length x nw floats each); host
+ * arrays of nseries device pointers.  Synchronous. */
+int rt_batch_bind(void* handle, const float* const* h_inputs, float* const* h_snrs);
+
+/* Enqueues the whole batch on `stream` without host synchronisation.  With
+ * use_graph the launch sequence is captured once into a hipGraph and
+ * replayed on later calls (until the next rt_batch_bind). */
+int rt_batch_run(void* handle, int use_graph, void* stream);
+
+/* One eager run with HIP events around each phase of each group, on
+ * `stream`; waits.  ms[0..3] = preparation ms, downsampling ms, FFA + S/N ms,
+ * number of FFA launches. */
+int rt_batch_run_timed(void* handle, void* stream, double* ms);
+
+/* Waits for `stream` and reports a device-side tiling error, if any. */
+int rt_batch_check(void* handle, void* stream);
+
+/* stats[8]: transforms, groups, workgroup slots, scratch bytes, packed
+ * downsampling bytes, folded input floats, trial periods, FFA float adds. */
+int rt_batch_stats(void* handle, int64_t* stats);
+
+int rt_batch_destroy(void* handle);
+
 #ifdef __cplusplus
 }
 #endif

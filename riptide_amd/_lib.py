@@ -42,6 +42,16 @@ _SIGNATURES = {
     "rt_deredden": [_c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_fast_running_median": [_c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_normalise": [_c_ptr, _c_i64, _c_ptr, _c_ptr],
+    "rt_batch_create": [_c_i64, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_dbl, _c_dbl, _c_ptr, _c_ptr,
+                        ctypes.c_int, _c_dbl, _c_i64, ctypes.c_int, _c_i64, _c_ptr],
+    "rt_batch_length": [_c_ptr, _c_i64, _c_ptr],
+    "rt_batch_grid": [_c_ptr, _c_i64, _c_ptr, _c_ptr],
+    "rt_batch_bind": [_c_ptr, _c_ptr, _c_ptr],
+    "rt_batch_run": [_c_ptr, ctypes.c_int, _c_ptr],
+    "rt_batch_run_timed": [_c_ptr, _c_ptr, _c_ptr],
+    "rt_batch_check": [_c_ptr, _c_ptr],
+    "rt_batch_stats": [_c_ptr, _c_ptr],
+    "rt_batch_destroy": [_c_ptr],
 }
 
 _lib = None

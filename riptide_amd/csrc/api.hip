@@ -28,6 +28,8 @@ static int fail(int code, const std::string& msg)
     return code;
 }
 
+void set_error(const std::string& msg) { g_err = msg; }
+
 #define RT_HIP(expr)                                                                  \
     do {                                                                              \
         hipError_t e_ = (expr);                                                       \

@@ -1,0 +1,563 @@
+// Batched DM-trial search: many series (uneven lengths, per-series bins
+// ranges) searched with a handful of launches per group of series:
+//   prepare (dereddening + normalisation) -> one packed downsampling launch
+//   for every step of every series -> one persistent FFA launch that drains
+//   a device queue of whole transforms, largest first.
+// A run only enqueues kernels with fixed arguments, so it can be captured
+// once in a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/riptide_amd.h"
+#include "ffa_kernels.h"
+#include "ffa_plan.h"
+#include "plan.h"
+#include "series_kernels.h"
+#include "tasks.h"
+
+namespace rt {
+
+void set_error(const std::string& msg);
+
+namespace {
+
+int bfail(int code, const std::string& msg)
+{
+    set_error(msg);
+    return code;
+}
+
+#define BT_HIP(expr)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (expr);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return bfail(RT_EDEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+__global__ void fill_int_kernel(int* p, int n, int v)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+struct Group {
+    int s0 = 0, s1 = 0;            // series [s0, s1)
+    int tr0 = 0, ntr = 0;          // transforms in the device table
+    int ds0 = 0;                   // DsSeriesDev entries [ds0, ds0 + s1 - s0)
+    int total_chunks = 0;
+    int halo = 2;
+    int nds_series = 0;            // series with at least one packed step
+};
+
+}  // namespace
+
+struct Batch {
+    int device = 0;
+    int64_t nseries = 0;
+    double pmin = 0, pmax = 0;
+    // per series
+    std::vector<int64_t> N;
+    std::vector<double> tsamp;
+    std::vector<int64_t> bmin, bmax;
+    std::vector<int> plan_of;            // index into plans
+    std::vector<int> wgroup_of;          // index into width groups
+    std::vector<int64_t> ds_off;         // offset in the group ds buffer
+    std::vector<int64_t> prep_off;       // offset in the group prepared buffer
+    std::vector<SeriesPlan> plans;
+    // width groups: (bmin, bmax, widths) -> tables
+    std::vector<std::vector<uint64_t>> wlists;
+    std::vector<int64_t> wg_bmin, wg_bmax;
+    std::vector<int64_t> wg_hb_off, wg_w_off;
+    // options
+    int deredden = 0, normalise = 0;
+    double rmed_width = 4.0;
+    int64_t rmed_minpts = 101;
+    // groups
+    std::vector<Group> groups;
+    std::vector<FfaTransform> host_trs;  // all groups, src/snr filled at bind
+    std::vector<int> tr_series;
+    std::vector<int> tr_plan_index;      // transform index within its plan
+    std::vector<DsSeriesDev> host_ds;
+    std::vector<DsStepDev> host_steps;
+    // device memory
+    float* d_tables = nullptr;
+    FfaTransform* d_trs = nullptr;
+    DsSeriesDev* d_ds_series = nullptr;
+    DsStepDev* d_ds_steps = nullptr;
+    float* d_ds = nullptr;
+    float* d_prep = nullptr;
+    float* d_scratch = nullptr;
+    float* d_work = nullptr;             // dereddening + normalisation scratch
+    int* d_queue = nullptr;
+    int* d_err = nullptr;
+    int64_t ds_floats = 0, prep_floats = 0, slot_floats = 0, work_floats = 0;
+    int slots = 0;
+    // bound buffers
+    std::vector<const float*> inputs;
+    std::vector<float*> outputs;
+    bool bound = false;
+    // graph
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t graph_stream = nullptr;
+};
+
+static void free_batch(Batch* b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    for (void* p : {(void*)b->d_tables, (void*)b->d_trs, (void*)b->d_ds_series,
+                    (void*)b->d_ds_steps, (void*)b->d_ds, (void*)b->d_prep,
+                    (void*)b->d_scratch, (void*)b->d_work, (void*)b->d_queue, (void*)b->d_err})
+        if (p) hipFree(p);
+    delete b;
+}
+
+static int64_t transform_cost(const Transform& t)
+{
+    return (int64_t)t.rows * t.bins * (ffa_height(t.rows) + 4);
+}
+
+static int scrunch_factor(int64_t width_samples, int64_t minpts)
+{
+    return (int)std::max(1.0, width_samples / (double)minpts);
+}
+
+}  // namespace rt
+
+using namespace rt;
+
+extern "C" {
+
+int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
+                    const uint64_t* widths_flat, const int64_t* widths_count, double period_min,
+                    double period_max, const int64_t* bins_min, const int64_t* bins_max,
+                    int deredden, double rmed_width, int64_t rmed_minpts, int normalise,
+                    int64_t group_size, void** out)
+{
+    if (nseries < 1) return bfail(RT_EINVAL, "batch needs at least one series");
+    if (deredden && !(rmed_minpts % 2)) return bfail(RT_EINVAL, "min_points must be an odd number");
+    Batch* b = new Batch();
+    BT_HIP(hipGetDevice(&b->device));
+    b->nseries = nseries;
+    b->pmin = period_min;
+    b->pmax = period_max;
+    b->deredden = deredden;
+    b->normalise = normalise;
+    b->rmed_width = rmed_width;
+    b->rmed_minpts = rmed_minpts;
+
+    // plans (shared by identical search parameters) and width groups
+    std::map<std::tuple<int64_t, double, int64_t, int64_t>, int> plan_ids;
+    std::map<std::tuple<int64_t, int64_t, std::vector<uint64_t>>, int> wg_ids;
+    const uint64_t* wp = widths_flat;
+    for (int64_t i = 0; i < nseries; ++i) {
+        b->N.push_back(sizes[i]);
+        b->tsamp.push_back(tsamps[i]);
+        b->bmin.push_back(bins_min[i]);
+        b->bmax.push_back(bins_max[i]);
+        const auto key = std::make_tuple(sizes[i], tsamps[i], bins_min[i], bins_max[i]);
+        auto it = plan_ids.find(key);
+        if (it == plan_ids.end()) {
+            SeriesPlan p;
+            const std::string e = plan_series(sizes[i], tsamps[i], period_min, period_max,
+                                              bins_min[i], bins_max[i], &p);
+            if (!e.empty()) {
+                free_batch(b);
+                return bfail(RT_EINVAL, e);
+            }
+            it = plan_ids.emplace(key, (int)b->plans.size()).first;
+            b->plans.push_back(std::move(p));
+        }
+        b->plan_of.push_back(it->second);
+        std::vector<uint64_t> w(wp, wp + widths_count[i]);
+        wp += widths_count[i];
+        for (uint64_t x : w)
+            if (!(x > 0 && x < (uint64_t)bins_min[i])) {
+                free_batch(b);
+                return bfail(RT_EINVAL, "trial widths must be all > 0 and < columns");
+            }
+        const auto wkey = std::make_tuple(bins_min[i], bins_max[i], w);
+        auto wt = wg_ids.find(wkey);
+        if (wt == wg_ids.end()) {
+            wt = wg_ids.emplace(wkey, (int)b->wlists.size()).first;
+            b->wlists.push_back(w);
+            b->wg_bmin.push_back(bins_min[i]);
+            b->wg_bmax.push_back(bins_max[i]);
+        }
+        b->wgroup_of.push_back(wt->second);
+        if (deredden) {
+            const int64_t ws = (int64_t)std::llround(rmed_width / tsamps[i]);
+            const int sf = scrunch_factor(ws, rmed_minpts);
+            const int64_t nl = sizes[i] / sf;
+            if (sf == 1 ? !(ws % 2 && ws < sizes[i]) : !(rmed_minpts < nl)) {
+                free_batch(b);
+                return bfail(RT_EINVAL, "running median width must be odd and < size");
+            }
+        }
+    }
+
+    // S/N tables of every width group: (h + b, b) per bins, then widths
+    std::vector<float> tables;
+    for (size_t g = 0; g < b->wlists.size(); ++g) {
+        const auto& w = b->wlists[g];
+        b->wg_hb_off.push_back((int64_t)tables.size());
+        for (int64_t bins = b->wg_bmin[g]; bins <= b->wg_bmax[g]; ++bins)
+            for (uint64_t x : w) {
+                float hpb, bb;
+                boxcar_hb(bins, (int64_t)x, &hpb, &bb);
+                tables.push_back(hpb);
+                tables.push_back(bb);
+            }
+        b->wg_w_off.push_back((int64_t)tables.size());
+        for (uint64_t x : w) {
+            const int32_t v = (int32_t)x;
+            float f;
+            std::memcpy(&f, &v, sizeof(f));
+            tables.push_back(f);
+        }
+    }
+
+    // groups of series; per group: packed downsampling descriptors and the
+    // transform queue sorted by decreasing cost
+    if (group_size < 1) group_size = 64;
+    int64_t max_ds = 1, max_prep = 1, max_fold = 1, max_work = 1;
+    b->ds_off.assign(nseries, 0);
+    b->prep_off.assign(nseries, 0);
+    for (int64_t s0 = 0; s0 < nseries; s0 += group_size) {
+        Group g;
+        g.s0 = (int)s0;
+        g.s1 = (int)std::min<int64_t>(nseries, s0 + group_size);
+        g.ds0 = (int)b->host_ds.size();
+        int64_t ds_run = 0, prep_run = 0;
+        double fmax = 1.0;
+        std::vector<std::pair<int64_t, int>> order;  // (cost, transform index in host_trs)
+        const int tr_base = (int)b->host_trs.size();
+        std::vector<FfaTransform> local;
+        std::vector<int> local_series, local_pidx;
+        for (int s = g.s0; s < g.s1; ++s) {
+            const SeriesPlan& p = b->plans[b->plan_of[s]];
+            b->ds_off[s] = ds_run;
+            b->prep_off[s] = prep_run;
+            prep_run += p.size;
+            DsSeriesDev d = {};
+            d.N = p.size;
+            d.step0 = (int32_t)b->host_steps.size();
+            d.chunk_offset = g.total_chunks;
+            int nst = 0;
+            for (const Step& st : p.steps) {
+                if (st.ds_offset < 0 || st.n == 0) continue;
+                if (st.f > kDsPackedMaxF) {
+                    free_batch(b);
+                    return bfail(RT_EINVAL, "downsampling factor too large for the batch path");
+                }
+                DsStepDev ds;
+                split_double(st.f, &ds.mf, &ds.ef);
+                ds.inv_f = (float)(1.0 / st.f);
+                ds.n = st.n;
+                ds.out_offset = st.ds_offset;  // relative to the series' region
+                b->host_steps.push_back(ds);
+                fmax = std::max(fmax, st.f);
+                ++nst;
+            }
+            if (nst > kDsMaxSteps) {
+                free_batch(b);
+                return bfail(RT_EINVAL, "too many downsampling steps");
+            }
+            d.nsteps = nst;
+            g.total_chunks += (int)((p.size + kDsChunk - 1) / kDsChunk);
+            b->host_ds.push_back(d);
+            ds_run += p.ds_total;
+            const int wg = b->wgroup_of[s];
+            for (size_t it = 0; it < p.tr.size(); ++it) {
+                const Transform& t = p.tr[it];
+                if (t.rows_eval <= 0) continue;
+                FfaTransform f = {};
+                f.bins = t.bins;
+                f.rows = t.rows;
+                f.rows_eval = t.rows_eval;
+                f.nw = (int32_t)b->wlists[wg].size();
+                f.stdnoise = t.stdnoise;
+                f.cap = ffa_cap_rows(t.bins, ffa_lds_bytes());
+                f.series = s;
+                if (t.rows <= f.cap) {
+                    f.npass = 1;
+                    f.levels0 = ffa_height(t.rows);
+                } else {
+                    const int np = count_transform_passes(t.rows, t.bins, f.cap);
+                    if (np < 0 || np > kMaxPasses) {
+                        free_batch(b);
+                        return bfail(RT_EINVAL, "bins too large for the FFA tiling");
+                    }
+                    std::vector<int> depths = transform_pass_depths(t.rows, f.cap);
+                    f.npass = (int32_t)depths.size();
+                    for (size_t q = 0; q < depths.size(); ++q) f.depth[q] = depths[q];
+                    max_fold = std::max<int64_t>(max_fold, (int64_t)t.rows * t.bins);
+                }
+                order.emplace_back(transform_cost(t), (int)local.size());
+                local.push_back(f);
+                local_series.push_back(s);
+                local_pidx.push_back((int)it);
+            }
+        }
+        std::stable_sort(order.begin(), order.end(),
+                         [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& c) {
+                             return a.first > c.first;
+                         });
+        for (const auto& o : order) {
+            b->host_trs.push_back(local[o.second]);
+            b->tr_series.push_back(local_series[o.second]);
+            b->tr_plan_index.push_back(local_pidx[o.second]);
+        }
+        g.tr0 = tr_base;
+        g.ntr = (int)order.size();
+        g.halo = (int)std::ceil(fmax) + 2;
+        max_ds = std::max(max_ds, ds_run);
+        max_prep = std::max(max_prep, prep_run);
+        b->groups.push_back(g);
+    }
+    for (int64_t i = 0; i < nseries; ++i) {
+        if (deredden) {
+            const int64_t ws = (int64_t)std::llround(rmed_width / tsamps[i]);
+            const int sf = scrunch_factor(ws, rmed_minpts);
+            max_work = std::max<int64_t>(max_work, sf == 1 ? sizes[i] : 2 * (sizes[i] / sf));
+        }
+    }
+    max_work += 2 * (kNormaliseParts + 2);
+
+    int cus = 0;
+    BT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
+    b->slots = std::max(1, cus);
+    b->ds_floats = max_ds;
+    b->prep_floats = (deredden || normalise) ? max_prep : 1;
+    b->slot_floats = 2 * max_fold;
+    b->work_floats = max_work;
+
+    BT_HIP(hipMalloc(&b->d_tables, std::max<size_t>(tables.size(), 1) * sizeof(float)));
+    if (!tables.empty())
+        BT_HIP(hipMemcpy(b->d_tables, tables.data(), tables.size() * sizeof(float),
+                         hipMemcpyHostToDevice));
+    BT_HIP(hipMalloc(&b->d_trs, std::max<size_t>(b->host_trs.size(), 1) * sizeof(FfaTransform)));
+    BT_HIP(hipMalloc(&b->d_ds_series, std::max<size_t>(b->host_ds.size(), 1) * sizeof(DsSeriesDev)));
+    BT_HIP(hipMalloc(&b->d_ds_steps, std::max<size_t>(b->host_steps.size(), 1) * sizeof(DsStepDev)));
+    BT_HIP(hipMalloc(&b->d_ds, (size_t)b->ds_floats * sizeof(float)));
+    BT_HIP(hipMalloc(&b->d_prep, (size_t)b->prep_floats * sizeof(float)));
+    BT_HIP(hipMalloc(&b->d_scratch, (size_t)b->slots * b->slot_floats * sizeof(float)));
+    BT_HIP(hipMalloc(&b->d_work, (size_t)b->work_floats * sizeof(float)));
+    BT_HIP(hipMalloc(&b->d_queue, std::max<size_t>(b->groups.size(), 1) * sizeof(int)));
+    BT_HIP(hipMalloc(&b->d_err, sizeof(int)));
+    BT_HIP(hipMemset(b->d_err, 0, sizeof(int)));
+    *out = b;
+    return RT_OK;
+}
+
+int rt_batch_length(void* handle, int64_t i, int64_t* length)
+{
+    Batch* b = (Batch*)handle;
+    if (i < 0 || i >= b->nseries) return bfail(RT_EINVAL, "series index out of range");
+    *length = b->plans[b->plan_of[i]].length;
+    return RT_OK;
+}
+
+int rt_batch_grid(void* handle, int64_t i, double* h_periods, uint32_t* h_foldbins)
+{
+    Batch* b = (Batch*)handle;
+    if (i < 0 || i >= b->nseries) return bfail(RT_EINVAL, "series index out of range");
+    plan_periods(b->plans[b->plan_of[i]], h_periods, h_foldbins);
+    return RT_OK;
+}
+
+int rt_batch_bind(void* handle, const float* const* h_inputs, float* const* h_snrs)
+{
+    Batch* b = (Batch*)handle;
+    b->inputs.assign(h_inputs, h_inputs + b->nseries);
+    b->outputs.assign(h_snrs, h_snrs + b->nseries);
+    const bool prep = b->deredden || b->normalise;
+    for (size_t k = 0; k < b->host_ds.size(); ++k) {
+        // host_ds is in series order
+        const int s = (int)k;
+        b->host_ds[k].x = prep ? b->d_prep + b->prep_off[s] : b->inputs[s];
+        b->host_ds[k].out = b->d_ds + b->ds_off[s];
+    }
+    for (size_t k = 0; k < b->host_trs.size(); ++k) {
+        FfaTransform& f = b->host_trs[k];
+        const int s = b->tr_series[k];
+        const SeriesPlan& p = b->plans[b->plan_of[s]];
+        const Transform& t = p.tr[b->tr_plan_index[k]];
+        const Step& st = p.steps[t.step];
+        const int wg = b->wgroup_of[s];
+        const float* base = prep ? b->d_prep + b->prep_off[s] : b->inputs[s];
+        f.src = (st.ds_offset < 0) ? base : b->d_ds + b->ds_off[s] + st.ds_offset;
+        f.snr = b->outputs[s] + t.out_row * f.nw;
+        f.widths = (const int32_t*)(b->d_tables + b->wg_w_off[wg]);
+        f.hb = b->d_tables + b->wg_hb_off[wg] + (t.bins - b->wg_bmin[wg]) * 2 * f.nw;
+    }
+    BT_HIP(hipMemcpy(b->d_trs, b->host_trs.data(), b->host_trs.size() * sizeof(FfaTransform),
+                     hipMemcpyHostToDevice));
+    BT_HIP(hipMemcpy(b->d_ds_series, b->host_ds.data(), b->host_ds.size() * sizeof(DsSeriesDev),
+                     hipMemcpyHostToDevice));
+    BT_HIP(hipMemcpy(b->d_ds_steps, b->host_steps.data(),
+                     b->host_steps.size() * sizeof(DsStepDev), hipMemcpyHostToDevice));
+    b->bound = true;
+    if (b->exec) {
+        hipGraphExecDestroy(b->exec);
+        hipGraphDestroy(b->graph);
+        b->exec = nullptr;
+        b->graph = nullptr;
+    }
+    return RT_OK;
+}
+
+// Optional per-phase event timing of an enqueue (events on the same stream).
+struct PhaseTimer {
+    std::vector<hipEvent_t> ev;  // 4 per group: start, after prep, after ds, after ffa
+};
+
+static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
+{
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        const Group& g = b->groups[gi];
+        if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 0], s));
+        // prepare: dereddening then normalisation, per series
+        if (b->deredden || b->normalise) {
+            for (int i = g.s0; i < g.s1; ++i) {
+                float* prep = b->d_prep + b->prep_off[i];
+                const float* x = b->inputs[i];
+                const int64_t N = b->N[i];
+                if (b->deredden) {
+                    const int64_t ws = (int64_t)std::llround(b->rmed_width / b->tsamp[i]);
+                    const int sf = scrunch_factor(ws, b->rmed_minpts);
+                    if (sf == 1) {
+                        BT_HIP(launch_running_median(x, N, (int)ws, b->d_work, s));
+                        BT_HIP(launch_deredden_subtract(x, N, b->d_work, N, 1, prep, s));
+                    } else {
+                        const int64_t nl = N / sf;
+                        float* lores = b->d_work;
+                        float* rmed = lores + nl;
+                        BT_HIP(launch_scrunch_mean(x, nl, sf, lores, s));
+                        BT_HIP(launch_running_median(lores, nl, (int)b->rmed_minpts, rmed, s));
+                        BT_HIP(launch_deredden_subtract(x, N, rmed, nl, sf, prep, s));
+                    }
+                    x = prep;
+                }
+                if (b->normalise) {
+                    float* work = b->d_work + (b->work_floats - 2 * (kNormaliseParts + 2));
+                    BT_HIP(launch_normalise(x, N, prep, work, kNormaliseParts, s));
+                }
+            }
+        }
+        if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 1], s));
+        if (g.total_chunks > 0)
+            BT_HIP(launch_downsample_packed(b->d_ds_series + g.s0, g.s1 - g.s0, b->d_ds_steps,
+                                            g.total_chunks, kDsChunk, g.halo, s));
+        hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, b->d_queue + gi, 1, 0);
+        BT_HIP(hipGetLastError());
+        if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 2], s));
+        BT_HIP(launch_ffa_batch(b->d_trs + g.tr0, g.ntr, b->d_queue + gi, b->d_scratch,
+                                b->slot_floats, b->slots, b->d_err, s));
+        if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 3], s));
+    }
+    return RT_OK;
+}
+
+int rt_batch_run(void* handle, int use_graph, void* stream)
+{
+    Batch* b = (Batch*)handle;
+    if (!b->bound) return bfail(RT_EINVAL, "rt_batch_bind must be called before rt_batch_run");
+    hipStream_t s = (hipStream_t)stream;
+    if (!use_graph) return enqueue_batch(b, s);
+    if (!b->exec) {
+        BT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int st = enqueue_batch(b, s);
+        hipGraph_t graph = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &graph);
+        if (st) {
+            if (graph) hipGraphDestroy(graph);
+            return st;
+        }
+        BT_HIP(e);
+        b->graph = graph;
+        BT_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    BT_HIP(hipGraphLaunch(b->exec, s));
+    return RT_OK;
+}
+
+int rt_batch_run_timed(void* handle, void* stream, double* ms)
+{
+    // ms[0] preparation, ms[1] downsampling, ms[2] FFA + S/N, ms[3] FFA launches
+    Batch* b = (Batch*)handle;
+    if (!b->bound) return bfail(RT_EINVAL, "rt_batch_bind must be called before rt_batch_run");
+    hipStream_t s = (hipStream_t)stream;
+    PhaseTimer pt;
+    pt.ev.resize(4 * b->groups.size());
+    for (auto& e : pt.ev) BT_HIP(hipEventCreate(&e));
+    int st = enqueue_batch(b, s, &pt);
+    if (st == RT_OK) {
+        BT_HIP(hipStreamSynchronize(s));
+        double acc[3] = {0, 0, 0};
+        for (size_t gi = 0; gi < b->groups.size(); ++gi)
+            for (int k = 0; k < 3; ++k) {
+                float t = 0;
+                BT_HIP(hipEventElapsedTime(&t, pt.ev[4 * gi + k], pt.ev[4 * gi + k + 1]));
+                acc[k] += t;
+            }
+        ms[0] = acc[0];
+        ms[1] = acc[1];
+        ms[2] = acc[2];
+        ms[3] = (double)b->groups.size();
+    }
+    for (auto& e : pt.ev) hipEventDestroy(e);
+    return st;
+}
+
+int rt_batch_check(void* handle, void* stream)
+{
+    Batch* b = (Batch*)handle;
+    int herr = 0;
+    BT_HIP(hipStreamSynchronize((hipStream_t)stream));
+    BT_HIP(hipMemcpy(&herr, b->d_err, sizeof(int), hipMemcpyDeviceToHost));
+    if (herr) return bfail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+    return RT_OK;
+}
+
+int rt_batch_stats(void* handle, int64_t* stats)
+{
+    // [0] transforms, [1] groups, [2] slots, [3] scratch bytes, [4] ds bytes,
+    // [5] fold input floats (sum rows*bins), [6] total periods, [7] FFA adds
+    Batch* b = (Batch*)handle;
+    int64_t fold = 0, periods = 0, adds = 0;
+    for (int64_t i = 0; i < b->nseries; ++i) {
+        const SeriesPlan& p = b->plans[b->plan_of[i]];
+        periods += p.length;
+        for (const Transform& t : p.tr) {
+            fold += (int64_t)t.rows * t.bins;
+            adds += ffa_adds(t.rows) * t.bins;
+        }
+    }
+    stats[0] = (int64_t)b->host_trs.size();
+    stats[1] = (int64_t)b->groups.size();
+    stats[2] = b->slots;
+    stats[3] = (int64_t)b->slots * b->slot_floats * 4;
+    stats[4] = b->ds_floats * 4;
+    stats[5] = fold;
+    stats[6] = periods;
+    stats[7] = adds;
+    return RT_OK;
+}
+
+int rt_batch_destroy(void* handle)
+{
+    if (handle) free_batch((Batch*)handle);
+    return RT_OK;
+}
+
+}  // extern "C"

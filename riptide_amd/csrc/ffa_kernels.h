@@ -15,6 +15,11 @@ constexpr int kFfaMaxE = 40;
 // if a task's cone would overrun LDS (never, for tasks from ffa_plan).
 hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipStream_t stream);
 
+// Persistent batch kernel over whole transforms: `slots` workgroups, each with
+// slot_floats floats of scratch (two intermediate states); *d_queue must be 0.
+hipError_t launch_ffa_batch(const FfaTransform* d_trs, int ntr, int* d_queue, float* d_scratch,
+                            int64_t slot_floats, int slots, int* d_err, hipStream_t stream);
+
 // Boxcar S/N of rows x cols profiles; long_scratch (rows*cols floats) is only
 // used for cols > 1024.
 hipError_t launch_snr_rows(const float* x, int64_t rows, int cols, const int32_t* widths,

@@ -186,14 +186,10 @@ __device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int b
     *sh = (s - tt) % bins;
 }
 
+// Runs one FfaTask with the whole workgroup; LDS contents are dead on return.
 template <int MAXE>
-__global__ __launch_bounds__(kFfaThreads) void ffa_task_kernel(const FfaTask* __restrict__ tasks,
-                                                                int ntasks, int* err)
+__device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 {
-    extern __shared__ float lds[];
-    const int task_id = blockIdx.x;
-    if (task_id >= ntasks) return;
-    const FfaTask tk = tasks[task_id];
     const int bins = tk.bins;
     const int tid = threadIdx.x;
     const int cap_rows = tk.pad;
@@ -213,6 +209,7 @@ __global__ __launch_bounds__(kFfaThreads) void ffa_task_kernel(const FfaTask* __
         if (rows > cap_rows) {
             // the host sizes tiles so this cannot happen; never overrun LDS
             if (tid == 0) atomicOr(err, 1);
+            __syncthreads();
             return;
         }
         // global source row of every bottom row
@@ -311,6 +308,7 @@ __global__ __launch_bounds__(kFfaThreads) void ffa_task_kernel(const FfaTask* __
         const int total = rows * bins;
         for (int e = tid; e < total; e += blockDim.x)
             dst[e] = data[e];
+        __syncthreads();
         return;
     }
 
@@ -343,6 +341,152 @@ __global__ __launch_bounds__(kFfaThreads) void ffa_task_kernel(const FfaTask* __
                 snr_widths_long(row, bins, lane, tk.widths, tk.hb, tk.nw, tk.stdnoise, out);
         }
         __syncthreads();
+    }
+}
+
+// One workgroup per task (the task list of one pass).
+template <int MAXE>
+__global__ __launch_bounds__(kFfaThreads) void ffa_task_kernel(const FfaTask* __restrict__ tasks,
+                                                                int ntasks, int* err)
+{
+    extern __shared__ float lds[];
+    if ((int)blockIdx.x >= ntasks) return;
+    const FfaTask tk = tasks[blockIdx.x];
+    run_ffa_task<MAXE>(tk, lds, err);
+}
+
+// Node i of depth d of an m-row merge tree (row order); false when that slot
+// lies under a leaf reached above depth d.
+__device__ __forceinline__ bool tree_node(int m, int d, int i, int* r0, int* nm)
+{
+    int a = 0, M = m;
+    for (int bit = d - 1; bit >= 0; --bit) {
+        if (M <= 1) {
+            if (i & ((1 << (bit + 1)) - 1)) return false;
+            break;
+        }
+        const int mh = M >> 1;
+        if ((i >> bit) & 1) {
+            a += mh;
+            M -= mh;
+        } else {
+            M = mh;
+        }
+    }
+    *r0 = a;
+    *nm = M;
+    return true;
+}
+
+// Next work item of a transform for the persistent kernel, in pass order:
+// pass 0 = whole nodes of depth depth[0] into buf_a, pass p >= 1 = cone tiles
+// of the nodes of depth depth[p], alternating buf_a -> buf_b -> buf_a ...,
+// the last pass producing S/N.  (p, i, lo) is the cursor.
+__device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int32_t* depth,
+                                              int& p, int& i, int& lo, float* buf_a,
+                                              float* buf_b, FfaTask& tk)
+{
+    for (;;) {
+        if (p >= tr.npass) return false;
+        if (tr.npass == 1) {
+            tk.src = tr.src;
+            tk.node_r0 = 0;
+            tk.node_m = tr.rows;
+            tk.lo = 0;
+            tk.hi = tr.rows;
+            tk.levels = tr.levels0;
+            tk.cone = 0;
+            tk.kind = 1;
+            p = 1;
+            return true;
+        }
+        const int d = depth[p];
+        if (i >= (1 << d)) {
+            ++p;
+            i = 0;
+            lo = 0;
+            continue;
+        }
+        int r0, nm;
+        if (!tree_node(tr.rows, d, i, &r0, &nm)) {
+            ++i;
+            lo = 0;
+            continue;
+        }
+        if (p == 0) {
+            int h = 0;
+            while ((1 << h) < nm) ++h;
+            tk.src = tr.src;
+            tk.dst = buf_a;
+            tk.node_r0 = r0;
+            tk.node_m = nm;
+            tk.lo = 0;
+            tk.hi = nm;
+            tk.levels = h;
+            tk.cone = 0;
+            tk.kind = 0;
+            ++i;
+            return true;
+        }
+        const int L = depth[p - 1] - d;
+        const int K = tr.cap - 2 * ((1 << L) - 1);
+        const bool last = (p == tr.npass - 1);
+        if (lo >= nm || (last && lo >= tr.rows_eval)) {
+            ++i;
+            lo = 0;
+            continue;
+        }
+        tk.src = (p & 1) ? buf_a : buf_b;
+        tk.dst = (p & 1) ? buf_b : buf_a;
+        tk.node_r0 = r0;
+        tk.node_m = nm;
+        tk.lo = lo;
+        tk.hi = min(lo + K, nm);
+        tk.levels = L;
+        tk.cone = 1;
+        tk.kind = last ? 1 : 0;
+        lo += K;
+        return true;
+    }
+}
+
+// Persistent batch kernel: every workgroup drains whole transforms from a
+// device queue (largest first) and runs all of a transform's passes itself,
+// its intermediate states in a private scratch slot, so no workgroup ever
+// waits on another.  Rows written by one pass are read by the next only by
+// this workgroup, ordered by the barrier that ends every task.
+template <int MAXE>
+__global__ __launch_bounds__(kFfaThreads) void ffa_batch_kernel(
+    const FfaTransform* __restrict__ trs, int ntr, int* __restrict__ queue, float* scratch,
+    int64_t slot_floats, int* err)
+{
+    extern __shared__ float lds[];
+    __shared__ int s_item;
+    float* buf_a = scratch + (size_t)blockIdx.x * slot_floats;
+    float* buf_b = buf_a + slot_floats / 2;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
+        __syncthreads();
+        // uniform across the workgroup: keep the descriptor in scalar registers
+        const int item = __builtin_amdgcn_readfirstlane(s_item);
+        __syncthreads();
+        if (item >= ntr) return;
+        const FfaTransform tr = trs[item];
+        FfaTask tk = {};
+        tk.bins = tr.bins;
+        tk.widths = tr.widths;
+        tk.hb = tr.hb;
+        tk.nw = tr.nw;
+        tk.stdnoise = tr.stdnoise;
+        tk.rows_eval = tr.rows_eval;
+        tk.snr = tr.snr;
+        tk.pad = tr.cap;
+        int p = 0, i = 0, lo = 0;
+        // the pass depths are indexed dynamically: read them from memory, so
+        // that the descriptor itself stays in registers
+        const int32_t* depth = trs[item].depth;
+        while (next_ffa_task(tr, depth, p, i, lo, buf_a, buf_b, tk))
+            run_ffa_task<MAXE>(tk, lds, err);
     }
 }
 
@@ -442,6 +586,23 @@ hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipS
     if (ntasks <= 0) return hipSuccess;
     hipLaunchKernelGGL(ffa_task_kernel<kFfaMaxE>, dim3(ntasks), dim3(kFfaThreads),
                        ffa_lds_bytes(), stream, d_tasks, ntasks, d_err);
+    return hipGetLastError();
+}
+
+hipError_t launch_ffa_batch(const FfaTransform* d_trs, int ntr, int* d_queue, float* d_scratch,
+                            int64_t slot_floats, int slots, int* d_err, hipStream_t stream)
+{
+    static bool configured = false;
+    if (!configured) {
+        hipError_t e = hipFuncSetAttribute((const void*)ffa_batch_kernel<kFfaMaxE>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           ffa_lds_bytes());
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    if (ntr <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ffa_batch_kernel<kFfaMaxE>, dim3(slots), dim3(kFfaThreads),
+                       ffa_lds_bytes(), stream, d_trs, ntr, d_queue, d_scratch, slot_floats, d_err);
     return hipGetLastError();
 }
 

@@ -83,6 +83,19 @@ static std::vector<int> cone_depths(int d1, int cap)
     return best;
 }
 
+std::vector<int> transform_pass_depths(int64_t m, int cap)
+{
+    if (m <= cap) return {0};
+    return cone_depths(leaf_depth(m, cap), cap);
+}
+
+int64_t ffa_adds(int64_t m)
+{
+    if (m <= 1) return 0;
+    const int64_t mh = m >> 1;
+    return ffa_adds(mh) + ffa_adds(m - mh) + m;
+}
+
 int count_transform_passes(int64_t m, int bins, int cap)
 {
     (void)bins;

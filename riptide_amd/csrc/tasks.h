@@ -46,6 +46,28 @@ struct FfaTask {
     int32_t pad;
 };
 
+// One whole FFA transform for the persistent batch kernel: the pass split
+// (depth[0] = leaf depth of pass 0, ..., depth[npass-1] = 0) is precomputed by
+// the host planner; tiles and nodes are enumerated on the device.
+constexpr int kMaxPasses = 6;
+struct FfaTransform {
+    const float* src;       // folded input (rows*bins samples)
+    float* snr;             // S/N of row 0 (row s at snr + s*nw)
+    const int32_t* widths;
+    const float* hb;
+    int32_t bins;
+    int32_t rows;
+    int32_t rows_eval;
+    int32_t nw;
+    float stdnoise;
+    int32_t cap;            // rows per workgroup
+    int32_t npass;          // 1: one task computes everything
+    int32_t levels0;        // npass == 1: merge levels of the whole transform
+    int32_t depth[kMaxPasses];
+    int32_t series;         // owning series (bookkeeping)
+    int32_t pad;
+};
+
 // One downsampling step of one series for the packed multi-step kernel.
 struct DsStep {
     double f;           // downsampling factor (> 1)

@@ -123,6 +123,32 @@ int rt_fast_running_median(const float* d_in, int64_t size, int64_t width_sample
 /* TimeSeries.normalise (time_series.py:67-93): zero mean, unit variance. */
 int rt_normalise(const float* d_in, int64_t size, float* d_out, void* stream);
 
+/* ---- Adjoints (no counterpart in the reference, which has no gradients) -- */
+
+/* rt_normalise that also returns h_stats[4] = mean (hi, lo), std (hi, lo). */
+int rt_normalise_ex(const float* d_in, int64_t size, float* d_out, float* h_stats, void* stream);
+
+/* Transpose of normalisation: d_gx = (g - mean(g) - y*mean(g*y)) * inv_std. */
+int rt_normalise_backward(const float* d_g, const float* d_y, int64_t size, float inv_std,
+                          float* d_gx, void* stream);
+
+/* Transpose of the FFA transform (rt_ffa2): d_gin = A^T d_gout. */
+int rt_ffa2_backward(const float* d_gout, int64_t rows, int64_t cols, float* d_gin, void* stream);
+
+/* Gradient of sum(gsnr * snr2(x)) with respect to x (rows x cols), through
+ * the first argmax window of every width. */
+int rt_snr2_backward(const float* d_in, int64_t rows, int64_t cols, const uint64_t* h_widths,
+                     int64_t nw, float stdnoise, const float* d_gsnr, float* d_gin, void* stream);
+
+/* Gradient of sum(gsnr * periodogram(x)) with respect to the (normalised)
+ * series x, recomputing the forward of every (step, bins) transform whose
+ * rows receive gradient: S/N backward -> FFA transpose -> fold transpose ->
+ * downsampling transpose.  d_gsnr is length x nw, d_gin has size floats. */
+int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
+                            const uint64_t* h_widths, int64_t nw, double period_min,
+                            double period_max, int64_t bins_min, int64_t bins_max,
+                            const float* d_gsnr, float* d_gin, void* stream);
+
 /* ---- Batched DM-trial search (no counterpart in the reference: it replaces
  * the reference's one-process-per-DM-trial pool, pipeline/worker_pool.py:35-45,
  * each worker calling ffa_search, search.py:10-93) -------------------------- */

@@ -42,6 +42,12 @@ _SIGNATURES = {
     "rt_deredden": [_c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_fast_running_median": [_c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_normalise": [_c_ptr, _c_i64, _c_ptr, _c_ptr],
+    "rt_normalise_ex": [_c_ptr, _c_i64, _c_ptr, _c_ptr, _c_ptr],
+    "rt_normalise_backward": [_c_ptr, _c_ptr, _c_i64, _c_flt, _c_ptr, _c_ptr],
+    "rt_ffa2_backward": [_c_ptr, _c_i64, _c_i64, _c_ptr, _c_ptr],
+    "rt_snr2_backward": [_c_ptr, _c_i64, _c_i64, _c_ptr, _c_i64, _c_flt, _c_ptr, _c_ptr, _c_ptr],
+    "rt_periodogram_backward": [_c_ptr, _c_i64, _c_dbl, _c_ptr, _c_i64, _c_dbl, _c_dbl, _c_i64,
+                                _c_i64, _c_ptr, _c_ptr, _c_ptr],
     "rt_batch_create": [_c_i64, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_dbl, _c_dbl, _c_ptr, _c_ptr,
                         ctypes.c_int, _c_dbl, _c_i64, ctypes.c_int, _c_i64, _c_ptr],
     "rt_batch_length": [_c_ptr, _c_i64, _c_ptr],

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include "../../include/riptide_amd.h"
+#include "adjoint.h"
 #include "exact_f64.h"
 #include "ffa_kernels.h"
 #include "ffa_plan.h"
@@ -681,6 +682,228 @@ int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const ui
     for (auto& ps : per_step)
         for (auto& p : ps) flat.push_back(p);
     return run_passes(flat, s, true);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Adjoints
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int rt_normalise_ex(const float* d_in, int64_t size, float* d_out, float* h_stats, void* stream)
+{
+    if (size < 1) return fail(RT_EINVAL, "cannot normalise an empty series");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    void* p = nullptr;
+    int st = pool_get(kSlotMisc, (size_t)(2 * (kNormaliseParts + 2)) * sizeof(float), &p);
+    if (st) return st;
+    RT_HIP(launch_normalise(d_in, size, d_out, (float*)p, kNormaliseParts, s));
+    // mean (hi, lo) and std (hi, lo) sit after the partials
+    RT_HIP(hipMemcpyAsync(h_stats, (float*)p + 2 * kNormaliseParts, 4 * sizeof(float),
+                          hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_normalise_backward(const float* d_g, const float* d_y, int64_t size, float inv_std,
+                          float* d_gx, void* stream)
+{
+    if (size < 1) return fail(RT_EINVAL, "empty series");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    void* p = nullptr;
+    const int parts = 256;
+    int st = pool_get(kSlotMisc, (size_t)(4 * parts) * sizeof(float), &p);
+    if (st) return st;
+    RT_HIP(launch_normalise_adjoint(d_g, d_y, size, inv_std, (float*)p, parts, d_gx, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_ffa2_backward(const float* d_gout, int64_t rows, int64_t cols, float* d_gin, void* stream)
+{
+    if (rows < 1 || cols < 1)
+        return fail(RT_EINVAL, "ffa2 input must have at least one row and one column");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    const int H = ffa_height(rows);
+    const size_t n = (size_t)(rows * cols);
+    if (H == 0) {
+        RT_HIP(hipMemcpyAsync(d_gin, d_gout, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    } else {
+        void* tmp = nullptr;
+        int st = pool_get(kSlotInter, n * sizeof(float), &tmp);
+        if (st) return st;
+        const float* src = d_gout;
+        for (int d = 0; d < H; ++d) {
+            const bool to_out = ((H - 1 - d) % 2) == 0;
+            float* dst = to_out ? d_gin : (float*)tmp;
+            RT_HIP(launch_ffa_adjoint_level(src, dst, rows, cols, d, s));
+            src = dst;
+        }
+    }
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_snr2_backward(const float* d_in, int64_t rows, int64_t cols, const uint64_t* h_widths,
+                     int64_t nw, float stdnoise, const float* d_gsnr, float* d_gin, void* stream)
+{
+    int st = check_stdnoise(stdnoise);
+    if (st) return st;
+    st = check_widths(h_widths, nw, cols);
+    if (st) return st;
+    if (nw > kSnrBwdMaxW) return fail(RT_EINVAL, "at most 64 trial widths in the S/N backward");
+    if (rows == 0) return RT_OK;
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    std::vector<float> tab = hb_table(cols, 1, h_widths, nw);
+    const size_t woff = tab.size();
+    tab.resize(woff + (size_t)nw);
+    for (int64_t i = 0; i < nw; ++i) {
+        const int32_t w = (int32_t)h_widths[i];
+        std::memcpy(&tab[woff + (size_t)i], &w, sizeof(int32_t));
+    }
+    float* d_tab = nullptr;
+    st = upload(kSlotTables, tab, &d_tab, s);
+    if (st) return st;
+    RT_HIP(launch_snr_backward(d_in, (int)rows, (int)cols, d_gsnr, (const int32_t*)(d_tab + woff),
+                               d_tab, (int)nw, stdnoise, d_gin, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
+                            const uint64_t* h_widths, int64_t nw, double pmin, double pmax,
+                            int64_t bmin, int64_t bmax, const float* d_gsnr, float* d_gin,
+                            void* stream)
+{
+    if (nw > kSnrBwdMaxW) return fail(RT_EINVAL, "at most 64 trial widths in the S/N backward");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    SeriesPlan p;
+    const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &p);
+    if (!e.empty()) return fail(RT_EINVAL, e);
+    int st = check_widths(h_widths, nw, bmin);
+    if (st) return st;
+    RT_HIP(launch_fill_zero(d_gin, size, s));
+    if (p.length == 0 || nw == 0) {
+        RT_HIP(hipStreamSynchronize(s));
+        return RT_OK;
+    }
+    // which transforms receive any gradient: per-row flags from the host copy
+    std::vector<float> g((size_t)(p.length * nw));
+    RT_HIP(hipMemcpyAsync(g.data(), d_gsnr, g.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    std::vector<char> need_tr(p.tr.size(), 0), need_step(p.steps.size(), 0);
+    for (size_t it = 0; it < p.tr.size(); ++it) {
+        const Transform& t = p.tr[it];
+        const float* gt = g.data() + (size_t)t.out_row * nw;
+        for (int64_t q = 0; q < (int64_t)t.rows_eval * nw; ++q)
+            if (gt[q] != 0.0f) {
+                need_tr[it] = 1;
+                need_step[t.step] = 1;
+                break;
+            }
+    }
+    // S/N tables for bins in [bmin, bmax]
+    std::vector<float> tab = hb_table(bmin, bmax - bmin + 1, h_widths, nw);
+    const size_t woff = tab.size();
+    tab.resize(woff + (size_t)nw);
+    for (int64_t i = 0; i < nw; ++i) {
+        const int32_t w = (int32_t)h_widths[i];
+        std::memcpy(&tab[woff + (size_t)i], &w, sizeof(int32_t));
+    }
+    float* d_tab = nullptr;
+    st = upload(kSlotTables, tab, &d_tab, s);
+    if (st) return st;
+    // workspace: ds values + ds gradient (max n) and three transform buffers
+    int64_t max_fold = 1;
+    for (size_t it = 0; it < p.tr.size(); ++it)
+        if (need_tr[it]) max_fold = std::max<int64_t>(max_fold, (int64_t)p.tr[it].rows * p.tr[it].bins);
+    void* wsp = nullptr;
+    st = pool_get(kSlotInter, (size_t)(2 * p.max_n + 4 * max_fold) * sizeof(float), &wsp);
+    if (st) return st;
+    float* ds = (float*)wsp;
+    float* gds = ds + p.max_n;
+    float* T = gds + p.max_n;
+    float* A = T + max_fold;
+    float* B = A + max_fold;
+    float* C = B + max_fold;
+    void* errp = nullptr;
+    st = pool_get(kSlotErr, sizeof(int), &errp);
+    if (st) return st;
+    RT_HIP(hipMemsetAsync(errp, 0, sizeof(int), s));
+
+    for (size_t is = 0; is < p.steps.size(); ++is) {
+        if (!need_step[is]) continue;
+        const Step& stp = p.steps[is];
+        const float* x = d_in;
+        if (stp.ds_offset >= 0) {
+            RT_HIP(launch_downsample_direct(d_in, size, stp.f, stp.n, ds, s));
+            x = ds;
+        }
+        RT_HIP(launch_fill_zero(gds, stp.n, s));
+        for (int it = stp.first_tr; it < stp.first_tr + stp.ntr; ++it) {
+            if (!need_tr[it]) continue;
+            const Transform& t = p.tr[it];
+            const int64_t n = (int64_t)t.rows * t.bins;
+            // forward transform rows (recomputed)
+            TransformSpec sp;
+            sp.m = t.rows;
+            sp.bins = t.bins;
+            sp.cap = ffa_cap_rows(t.bins, ffa_lds_bytes());
+            sp.src = x;
+            sp.kind = 0;
+            sp.dst = T;
+            sp.buf_a = A;
+            sp.buf_b = B;
+            std::vector<std::vector<FfaTask>> passes;
+            if (plan_transform_tasks(sp, passes) < 0)
+                return fail(RT_EINVAL, "bins too large for the FFA tiling");
+            std::vector<FfaTask> all;
+            for (auto& q : passes) all.insert(all.end(), q.begin(), q.end());
+            FfaTask* d_tasks = nullptr;
+            st = upload(kSlotTasks, all, &d_tasks, s);
+            if (st) return st;
+            size_t off = 0;
+            for (auto& q : passes) {
+                RT_HIP(launch_ffa_tasks(d_tasks + off, (int)q.size(), (int*)errp, s));
+                off += q.size();
+            }
+            // dL/d(transform rows): S/N backward on the evaluated rows, zero beyond
+            RT_HIP(launch_fill_zero(A, n, s));
+            RT_HIP(launch_snr_backward(T, t.rows_eval, t.bins, d_gsnr + t.out_row * nw,
+                                       (const int32_t*)(d_tab + woff),
+                                       d_tab + (size_t)(t.bins - bmin) * nw * 2, (int)nw,
+                                       t.stdnoise, A, s));
+            // FFA transpose, depth 0 .. H-1
+            const int H = ffa_height(t.rows);
+            const float* src = A;
+            float* bufs[2] = {B, C};
+            for (int d = 0; d < H; ++d) {
+                float* dst = bufs[d & 1];
+                RT_HIP(launch_ffa_adjoint_level(src, dst, t.rows, t.bins, d, s));
+                src = dst;
+            }
+            // fold transpose: the first rows*bins samples, accumulated
+            RT_HIP(launch_add_prefix(src, n, stp.n, gds, 1, s));
+            // the task list upload of the next transform reuses the slot
+            RT_HIP(hipStreamSynchronize(s));
+        }
+        if (stp.ds_offset >= 0)
+            RT_HIP(launch_downsample_adjoint(gds, stp.n, stp.f, size, d_gin, 1, s));
+        else
+            RT_HIP(launch_add_prefix(gds, stp.n, size, d_gin, 1, s));
+    }
+    int herr = 0;
+    RT_HIP(hipMemcpyAsync(&herr, errp, sizeof(int), hipMemcpyDeviceToHost, s));
+    RT_HIP(hipStreamSynchronize(s));
+    if (herr) return fail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+    return RT_OK;
 }
 
 }  // extern "C"

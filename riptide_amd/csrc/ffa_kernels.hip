@@ -226,6 +226,8 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
             const int c = e - p * bins;
             data[e] = tk.src[(size_t)tab_a[p] + c];
         }
+        // the first level's table overwrites tab_a: every load must have read it
+        __syncthreads();
     } else {
         rows = tk.node_m;
         const float* src = tk.src + (size_t)tk.node_r0 * bins;

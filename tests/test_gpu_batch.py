@@ -27,7 +27,7 @@ def test_batch_matches_oracle(oracle):
     bmin = np.array([240 if i % 2 else 480 for i in range(10)])
     bmax = np.where(bmin == 240, 260, 520)
     data = [_series(rng, n, tsamp) for n in sizes]
-    b = BatchSearch(sizes, tsamp, bmin, bmax, 0.05, 2.0, rmed_width=1.0, group_size=4)
+    b = BatchSearch(sizes, tsamp, bmin, bmax, 0.08, 2.0, rmed_width=1.0, group_size=4)
     xs = [_lib.to_device(x) for x in data]
     outs = b.allocate_outputs()
     b.bind(xs, outs)
@@ -38,7 +38,7 @@ def test_batch_matches_oracle(oracle):
         b.check()
         torch.cuda.synchronize()
         for i, x in enumerate(data):
-            _, w, P, F, S = oracle.ffa_search(x, tsamp, period_min=0.05, period_max=2.0,
+            _, w, P, F, S = oracle.ffa_search(x, tsamp, period_min=0.08, period_max=2.0,
                                               bins_min=int(bmin[i]), bins_max=int(bmax[i]),
                                               rmed_width=1.0)
             Pb, Fb = b.grid(i)

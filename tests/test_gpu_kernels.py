@@ -66,7 +66,8 @@ def test_downsample_parity(lc, oracle, f):
     assert got.shape == want.shape
     # same index bounds and edge weights; only the float32 summation order
     # differs from the vectorised reference loop
-    np.testing.assert_allclose(got, want, rtol=0, atol=4e-7 * np.sqrt(f) * 8)
+    # float32 rounding of ~f sequential adds of O(sqrt(f)) partial sums
+    np.testing.assert_allclose(got, want, rtol=0, atol=3e-7 * f + 1e-6)
 
 
 @pytest.mark.parametrize("width", [1, 3, 5, 7, 11, 25, 37, 101])

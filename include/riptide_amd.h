@@ -37,6 +37,12 @@ const char* rt_last_error(void);
 /* Library ABI version. */
 int rt_version(void);
 
+/* Workgroup shape of the FFA kernels for subsequent calls: 0 = one
+ * 1024-thread workgroup per CU (~159 KiB LDS), 1 = two 512-thread workgroups
+ * per CU (~79 KiB each), -1 = the tuned default.  Results are identical;
+ * only speed differs (riptide_amd/tune.py picks the faster). */
+int rt_set_ffa_shape(int shape);
+
 /* ---- Host-side search plan (no device work) ------------------------------ */
 
 /* periodogram_length (periodogram.hpp:63-109; python_bindings.cpp:196):
@@ -104,6 +110,11 @@ int rt_circular_prefix_sum(const float* d_in, int64_t size, int64_t nsum, float*
 /* libcpp.benchmark_ffa2 (python_bindings.cpp:99-118): seconds per FFA of a
  * zero rows x cols block, timed on the device. */
 int rt_benchmark_ffa2(int64_t rows, int64_t cols, int64_t loops, double* seconds);
+
+/* Device milliseconds of one FFA transform of a rows x cols block (with the
+ * fused S/N of every row when with_snr) using workgroup shape `shape`,
+ * averaged over `reps` launches; used by the tile autotuner. */
+int rt_time_ffa(int64_t rows, int64_t cols, int shape, int with_snr, int64_t reps, double* ms);
 
 /* ---- Python-side numerics of the path (numpy in the reference) ----------- */
 

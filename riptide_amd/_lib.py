@@ -23,6 +23,7 @@ _c_ptr = ctypes.c_void_p
 # name -> argtypes (all return int status)
 _SIGNATURES = {
     "rt_version": [],
+    "rt_set_ffa_shape": [ctypes.c_int],
     "rt_periodogram_length": [_c_i64, _c_dbl, _c_dbl, _c_dbl, _c_i64, _c_i64, _c_ptr],
     "rt_periodogram_grid": [_c_i64, _c_dbl, _c_dbl, _c_dbl, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_downsample_index": [_c_i64, _c_dbl, _c_i64, _c_ptr, _c_ptr, _c_ptr, _c_ptr],
@@ -38,6 +39,7 @@ _SIGNATURES = {
     "rt_fused_rollback_add": [_c_ptr, _c_ptr, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_circular_prefix_sum": [_c_ptr, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_benchmark_ffa2": [_c_i64, _c_i64, _c_i64, _c_ptr],
+    "rt_time_ffa": [_c_i64, _c_i64, ctypes.c_int, ctypes.c_int, _c_i64, _c_ptr],
     "rt_scrunch_mean": [_c_ptr, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_deredden": [_c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_fast_running_median": [_c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr],

@@ -31,6 +31,10 @@ static int fail(int code, const std::string& msg)
 
 void set_error(const std::string& msg) { g_err = msg; }
 
+// Workgroup shape of the FFA kernels for calls made from now on.
+static int g_ffa_shape = -1;
+int current_ffa_shape() { return g_ffa_shape < 0 ? ffa_default_shape() : g_ffa_shape; }
+
 #define RT_HIP(expr)                                                                  \
     do {                                                                              \
         hipError_t e_ = (expr);                                                       \
@@ -136,7 +140,7 @@ static std::vector<float> hb_table(int64_t b0, int64_t nb, const uint64_t* w, in
 // Uploads all passes' tasks, launches them in order, waits and checks the
 // kernel's overflow flag.
 static int run_passes(const std::vector<std::vector<FfaTask>>& passes, hipStream_t stream,
-                      bool sync)
+                      bool sync, int shape)
 {
     std::vector<FfaTask> all;
     std::vector<size_t> first;
@@ -153,7 +157,8 @@ static int run_passes(const std::vector<std::vector<FfaTask>>& passes, hipStream
     if (st) return st;
     RT_HIP(hipMemsetAsync(errp, 0, sizeof(int), stream));
     for (size_t i = 0; i < passes.size(); ++i)
-        RT_HIP(launch_ffa_tasks(d_tasks + first[i], (int)passes[i].size(), (int*)errp, stream));
+        RT_HIP(launch_ffa_tasks(d_tasks + first[i], (int)passes[i].size(), (int*)errp, stream,
+                                shape));
     if (sync) {
         int herr = 0;
         RT_HIP(hipMemcpyAsync(&herr, errp, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -202,6 +207,13 @@ const char* rt_last_error(void) { return g_err.c_str(); }
 
 int rt_version(void) { return 1; }
 
+int rt_set_ffa_shape(int shape)
+{
+    if (shape < -1 || shape >= kFfaShapes) return fail(RT_EINVAL, "unknown FFA workgroup shape");
+    g_ffa_shape = shape;
+    return RT_OK;
+}
+
 int rt_periodogram_length(int64_t size, double tsamp, double pmin, double pmax, int64_t bmin,
                           int64_t bmax, int64_t* length)
 {
@@ -245,7 +257,8 @@ int rt_ffa2(const float* d_in, int64_t rows, int64_t cols, float* d_out, void* s
         return fail(RT_EINVAL, "ffa2 input too large");
     std::lock_guard<std::mutex> lk(g_pool_mu);
     hipStream_t s = S(stream);
-    const int cap = ffa_cap_rows((int)std::min<int64_t>(cols, 1 << 20), ffa_lds_bytes());
+    const int shape = current_ffa_shape();
+    const int cap = ffa_cap_rows((int)std::min<int64_t>(cols, 1 << 20), shape);
     if (cols > (1 << 20) || cap < kMinCapRows || rows > (1LL << 31) / std::max<int64_t>(cols, 1)) {
         const int st = ffa_global(d_in, rows, cols, d_out, s);
         if (st) return st;
@@ -256,6 +269,7 @@ int rt_ffa2(const float* d_in, int64_t rows, int64_t cols, float* d_out, void* s
     sp.m = rows;
     sp.bins = (int)cols;
     sp.cap = cap;
+    sp.shape = shape;
     sp.src = d_in;
     sp.kind = 0;
     sp.dst = d_out;
@@ -274,7 +288,85 @@ int rt_ffa2(const float* d_in, int64_t rows, int64_t cols, float* d_out, void* s
         RT_HIP(hipStreamSynchronize(s));
         return RT_OK;
     }
-    return run_passes(passes, s, true);
+    return run_passes(passes, s, true, shape);
+}
+
+int rt_time_ffa(int64_t rows, int64_t cols, int shape, int with_snr, int64_t reps, double* ms)
+{
+    // Device time of one FFA transform (+ S/N of every row, 10 widths) of a
+    // zero-filled rows x cols block with workgroup shape `shape`: planning and
+    // uploads happen once, then the passes are launched `reps` times between
+    // two events.
+    if (rows < 2 || cols < 2 || reps < 1) return fail(RT_EINVAL, "rows, cols >= 2 and reps >= 1");
+    if (shape < 0 || shape >= kFfaShapes) return fail(RT_EINVAL, "unknown FFA workgroup shape");
+    const int cap = ffa_cap_rows((int)cols, shape);
+    if (cap < kMinCapRows) return fail(RT_EINVAL, "rows too wide for the LDS path");
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    const size_t n = (size_t)(rows * cols);
+    float *in = nullptr, *out = nullptr, *inter = nullptr, *tab = nullptr;
+    int* err = nullptr;
+    FfaTask* d_tasks = nullptr;
+    RT_HIP(hipMalloc(&in, n * sizeof(float)));
+    RT_HIP(hipMalloc(&out, std::max<size_t>(n, (size_t)rows * 16) * sizeof(float)));
+    RT_HIP(hipMalloc(&inter, 2 * n * sizeof(float)));
+    RT_HIP(hipMemset(in, 0, n * sizeof(float)));
+    std::vector<uint64_t> w = {1, 2, 3, 4, 6, 9, 13, 19, 28, 42};
+    while (!w.empty() && w.back() >= (uint64_t)cols) w.pop_back();
+    std::vector<float> t = hb_table(cols, 1, w.data(), (int64_t)w.size());
+    const size_t woff = t.size();
+    t.resize(woff + w.size());
+    for (size_t i = 0; i < w.size(); ++i) {
+        const int32_t v = (int32_t)w[i];
+        std::memcpy(&t[woff + i], &v, sizeof(int32_t));
+    }
+    RT_HIP(hipMalloc(&tab, t.size() * sizeof(float)));
+    RT_HIP(hipMemcpy(tab, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+    RT_HIP(hipMalloc(&err, sizeof(int)));
+    RT_HIP(hipMemset(err, 0, sizeof(int)));
+    TransformSpec sp;
+    sp.m = rows;
+    sp.bins = (int)cols;
+    sp.cap = cap;
+    sp.shape = shape;
+    sp.src = in;
+    sp.buf_a = inter;
+    sp.buf_b = inter + n;
+    sp.kind = with_snr ? 1 : 0;
+    sp.dst = out;
+    sp.snr = out;
+    sp.widths = (const int32_t*)(tab + woff);
+    sp.hb = tab;
+    sp.nw = (int)w.size();
+    sp.stdnoise = 1.0f;
+    sp.rows_eval = (int)rows;
+    std::vector<std::vector<FfaTask>> passes;
+    if (plan_transform_tasks(sp, passes) < 0) return fail(RT_EINVAL, "cannot tile this block");
+    std::vector<FfaTask> all;
+    std::vector<size_t> first;
+    for (auto& q : passes) {
+        first.push_back(all.size());
+        all.insert(all.end(), q.begin(), q.end());
+    }
+    RT_HIP(hipMalloc(&d_tasks, all.size() * sizeof(FfaTask)));
+    RT_HIP(hipMemcpy(d_tasks, all.data(), all.size() * sizeof(FfaTask), hipMemcpyHostToDevice));
+    hipEvent_t a, b;
+    RT_HIP(hipEventCreate(&a));
+    RT_HIP(hipEventCreate(&b));
+    for (int64_t r = -1; r < reps; ++r) {
+        if (r == 0) RT_HIP(hipEventRecord(a, nullptr));
+        for (size_t i = 0; i < passes.size(); ++i)
+            RT_HIP(launch_ffa_tasks(d_tasks + first[i], (int)passes[i].size(), err, nullptr, shape));
+    }
+    RT_HIP(hipEventRecord(b, nullptr));
+    RT_HIP(hipEventSynchronize(b));
+    float t_ms = 0;
+    RT_HIP(hipEventElapsedTime(&t_ms, a, b));
+    *ms = t_ms / (double)reps;
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    for (void* q : {(void*)in, (void*)out, (void*)inter, (void*)tab, (void*)err, (void*)d_tasks})
+        hipFree(q);
+    return RT_OK;
 }
 
 int rt_snr2(const float* d_in, int64_t rows, int64_t cols, const uint64_t* h_widths, int64_t nw,
@@ -486,6 +578,7 @@ namespace rt {
 
 struct PgramSetup {
     SeriesPlan plan;
+    int shape = 0;
     float* d_hb = nullptr;
     int32_t* d_w = nullptr;
     float* d_ds = nullptr;
@@ -500,6 +593,7 @@ static int pgram_setup(const float* d_in, int64_t size, double tsamp, const uint
 {
     const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &out->plan);
     if (!e.empty()) return fail(RT_EINVAL, e);
+    out->shape = current_ffa_shape();
     const SeriesPlan& p = out->plan;
     int st = check_widths(h_widths, nw, bmin);
     if (st) return st;
@@ -591,7 +685,7 @@ static int64_t pgram_tasks(const PgramSetup& su, const float* d_in, int s0, int 
         for (int it = stp.first_tr; it < stp.first_tr + stp.ntr; ++it) {
             const Transform& t = p.tr[it];
             if (t.rows_eval <= 0) continue;
-            const int cap = ffa_cap_rows(t.bins, ffa_lds_bytes());
+            const int cap = ffa_cap_rows(t.bins, su.shape);
             const int64_t n = (int64_t)t.rows * t.bins;
             TransformSpec sp;
             sp.m = t.rows;
@@ -645,7 +739,7 @@ int rt_periodogram(const float* d_in, int64_t size, double tsamp, const uint64_t
     if (st) return st;
     if (pgram_tasks(su, d_in, 0, ns, d_snrs, (float*)inter, passes, false) < 0)
         return fail(RT_EINVAL, "bins too large for the FFA tiling");
-    return run_passes(passes, s, true);
+    return run_passes(passes, s, true, su.shape);
 }
 
 int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
@@ -681,7 +775,7 @@ int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const ui
     std::vector<std::vector<FfaTask>> flat;
     for (auto& ps : per_step)
         for (auto& p : ps) flat.push_back(p);
-    return run_passes(flat, s, true);
+    return run_passes(flat, s, true, su.shape);
 }
 
 }  // extern "C"
@@ -855,7 +949,9 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
             TransformSpec sp;
             sp.m = t.rows;
             sp.bins = t.bins;
-            sp.cap = ffa_cap_rows(t.bins, ffa_lds_bytes());
+            const int shape = current_ffa_shape();
+            sp.cap = ffa_cap_rows(t.bins, shape);
+            sp.shape = shape;
             sp.src = x;
             sp.kind = 0;
             sp.dst = T;
@@ -871,7 +967,7 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
             if (st) return st;
             size_t off = 0;
             for (auto& q : passes) {
-                RT_HIP(launch_ffa_tasks(d_tasks + off, (int)q.size(), (int*)errp, s));
+                RT_HIP(launch_ffa_tasks(d_tasks + off, (int)q.size(), (int*)errp, s, shape));
                 off += q.size();
             }
             // dL/d(transform rows): S/N backward on the evaluated rows, zero beyond

@@ -26,6 +26,7 @@
 namespace rt {
 
 void set_error(const std::string& msg);
+int current_ffa_shape();
 
 namespace {
 
@@ -100,6 +101,7 @@ struct Batch {
     int* d_err = nullptr;
     int64_t ds_floats = 0, prep_floats = 0, slot_floats = 0, work_floats = 0;
     int slots = 0;
+    int shape = 0;
     // bound buffers
     std::vector<const float*> inputs;
     std::vector<float*> outputs;
@@ -112,6 +114,7 @@ struct Batch {
 
 static void free_batch(Batch* b)
 {
+    if (b->graph_stream) hipStreamDestroy(b->graph_stream);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     for (void* p : {(void*)b->d_tables, (void*)b->d_trs, (void*)b->d_ds_series,
@@ -154,6 +157,7 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
     b->normalise = normalise;
     b->rmed_width = rmed_width;
     b->rmed_minpts = rmed_minpts;
+    b->shape = current_ffa_shape();
 
     // plans (shared by identical search parameters) and width groups
     std::map<std::tuple<int64_t, double, int64_t, int64_t>, int> plan_ids;
@@ -286,7 +290,8 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                 f.rows_eval = t.rows_eval;
                 f.nw = (int32_t)b->wlists[wg].size();
                 f.stdnoise = t.stdnoise;
-                f.cap = ffa_cap_rows(t.bins, ffa_lds_bytes());
+                f.cap = ffa_cap_rows(t.bins, b->shape);
+                f.shape = b->shape;
                 f.series = s;
                 if (t.rows <= f.cap) {
                     f.npass = 1;
@@ -335,7 +340,9 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
 
     int cus = 0;
     BT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-    b->slots = std::max(1, cus);
+    // workgroups resident at once: one per CU for the 1024-thread shape, two
+    // for the 512-thread shape
+    b->slots = std::max(1, cus) * (kFfaShape[b->shape].threads == 1024 ? 1 : 2);
     b->ds_floats = max_ds;
     b->prep_floats = (deredden || normalise) ? max_prep : 1;
     b->slot_floats = 2 * max_fold;
@@ -462,7 +469,7 @@ static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
         BT_HIP(hipGetLastError());
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 2], s));
         BT_HIP(launch_ffa_batch(b->d_trs + g.tr0, g.ntr, b->d_queue + gi, b->d_scratch,
-                                b->slot_floats, b->slots, b->d_err, s));
+                                b->slot_floats, b->slots, b->d_err, s, b->shape));
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 3], s));
     }
     return RT_OK;
@@ -475,10 +482,14 @@ int rt_batch_run(void* handle, int use_graph, void* stream)
     hipStream_t s = (hipStream_t)stream;
     if (!use_graph) return enqueue_batch(b, s);
     if (!b->exec) {
-        BT_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        const int st = enqueue_batch(b, s);
+        // capture on a private stream (the legacy default stream cannot be
+        // captured); the graph is then launched on the caller's stream
+        if (!b->graph_stream) BT_HIP(hipStreamCreateWithFlags(&b->graph_stream, hipStreamNonBlocking));
+        hipStream_t cs = b->graph_stream;
+        BT_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+        const int st = enqueue_batch(b, cs);
         hipGraph_t graph = nullptr;
-        const hipError_t e = hipStreamEndCapture(s, &graph);
+        const hipError_t e = hipStreamEndCapture(cs, &graph);
         if (st) {
             if (graph) hipGraphDestroy(graph);
             return st;

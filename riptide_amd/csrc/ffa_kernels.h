@@ -8,17 +8,16 @@
 
 namespace rt {
 
-// Register staging of the FFA task kernel: elements per thread per level.
-constexpr int kFfaMaxE = 40;
-
 // Runs ntasks FfaTask work items (one workgroup each).  *d_err gets bit 0 set
 // if a task's cone would overrun LDS (never, for tasks from ffa_plan).
-hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipStream_t stream);
+hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipStream_t stream,
+                            int shape);
 
 // Persistent batch kernel over whole transforms: `slots` workgroups, each with
 // slot_floats floats of scratch (two intermediate states); *d_queue must be 0.
 hipError_t launch_ffa_batch(const FfaTransform* d_trs, int ntr, int* d_queue, float* d_scratch,
-                            int64_t slot_floats, int slots, int* d_err, hipStream_t stream);
+                            int64_t slot_floats, int slots, int* d_err, hipStream_t stream,
+                            int shape);
 
 // Boxcar S/N of rows x cols profiles; long_scratch (rows*cols floats) is only
 // used for cols > 1024.

@@ -187,12 +187,22 @@ __device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int b
 }
 
 // Runs one FfaTask with the whole workgroup; LDS contents are dead on return.
-template <int MAXE>
+//
+// Every merge level is processed one row per wavefront: the row's head/tail
+// offsets and shift come from a per-row table (one broadcast LDS read each),
+// lanes walk the columns c = lane + 64*q, and each lane keeps up to kStage
+// results in registers until the barrier after which the level is written
+// back in place.
+template <int NT>
 __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 {
+    constexpr int NW = NT / kWave;
     const int bins = tk.bins;
     const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     const int cap_rows = tk.pad;
+    const int k = (bins + kWave - 1) / kWave;  // columns per lane
 
     float* data = lds;
     int* tab_a = reinterpret_cast<int*>(data + (size_t)cap_rows * bins);
@@ -214,17 +224,16 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
         }
         // global source row of every bottom row
         const int t = tk.levels;
-        for (int p = tid; p < rows; p += blockDim.x) {
+        for (int p = tid; p < rows; p += NT) {
             const int slot = seg_find(seg, t, p);
             const Seg g = seg[slot];
             tab_a[p] = (g.r0 + g.lo + (p - g.base)) * bins;
         }
         __syncthreads();
-        const int total = rows * bins;
-        for (int e = tid; e < total; e += blockDim.x) {
-            const int p = e / bins;
-            const int c = e - p * bins;
-            data[e] = tk.src[(size_t)tab_a[p] + c];
+        for (int p = wave; p < rows; p += NW) {
+            const float* srow = tk.src + (size_t)__builtin_amdgcn_readfirstlane(tab_a[p]);
+            float* drow = data + p * bins;
+            for (int c = lane; c < bins; c += kWave) drow[c] = srow[c];
         }
         // the first level's table overwrites tab_a: every load must have read it
         __syncthreads();
@@ -232,7 +241,7 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
         rows = tk.node_m;
         const float* src = tk.src + (size_t)tk.node_r0 * bins;
         const int total = rows * bins;
-        for (int e = tid; e < total; e += blockDim.x)
+        for (int e = tid; e < total; e += NT)
             data[e] = src[e];
     }
 
@@ -240,7 +249,6 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
     for (int t = tk.levels - 1; t >= 0; --t) {
         int out_rows;
         if (tk.cone) {
-            out_rows = 0;
             const int first = seg_slot(t, 0);
             const int n = 1 << t;
             // rows of level t = sum of segment lengths (last base + len)
@@ -249,7 +257,7 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
             out_rows = rows;
         }
         // table of level t
-        for (int p = tid; p < out_rows; p += blockDim.x) {
+        for (int p = tid; p < out_rows; p += NT) {
             int a, b, sh;
             if (tk.cone)
                 cone_row_map(seg, t, p, bins, &a, &b, &sh);
@@ -260,45 +268,60 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
             tab_s[p] = sh;
         }
         __syncthreads();
-        // compute into registers
-        float v[MAXE];
-        const int total = out_rows * bins;
+        // compute: staged value jj of this lane is row wave + NW*i, column
+        // lane + 64*q, with (i, q) walking the rows of this wavefront
+        float v[kStage];
+        // the (row, column) walk below is the same on every level: hide lane
+        // and k from the optimiser so that it is not precomputed for all
+        // kStage steps outside the level loop
+        int ln = lane, kq = k;
+        asm volatile("" : "+v"(ln), "+s"(kq));
         {
-            int p = tid / bins;
-            int c = tid - p * bins;
-            // the walk below is the same on every level; keep the compiler
-            // from hoisting all MAXE (row, column) pairs out of the level loop
-            asm volatile("" : "+v"(p), "+v"(c));
-            const int dp = blockDim.x / bins;
-            const int dc = blockDim.x - dp * bins;
+            int i = 0, q = 0, a = 0, b = -1, sh = 0;
+            bool valid = false;
 #pragma unroll
-            for (int q = 0; q < MAXE; ++q) {
-                const int e = tid + q * blockDim.x;
-                if (e < total) {
-                    const int a = tab_a[p];
-                    const int b = tab_b[p];
-                    int cc = c + tab_s[p];
-                    cc = (cc >= bins) ? cc - bins : cc;
-                    const float x = data[a + c];
-                    const float y = (b >= 0) ? data[(b >= 0 ? b : 0) + cc] : -0.0f;
-                    v[q] = x + y;
+            for (int jj = 0; jj < kStage; ++jj) {
+                if (q == 0) {
+                    const int p = wave + i * NW;
+                    valid = p < out_rows;
+                    if (valid) {
+                        a = __builtin_amdgcn_readfirstlane(tab_a[p]);
+                        b = __builtin_amdgcn_readfirstlane(tab_b[p]);
+                        sh = __builtin_amdgcn_readfirstlane(tab_s[p]);
+                    }
                 }
-                p += dp;
-                c += dc;
-                if (c >= bins) {
-                    c -= bins;
-                    ++p;
+                if (valid) {
+                    const int c = ln + q * kWave;
+                    if (c < bins) {
+                        int cc = c + sh;
+                        cc = (cc >= bins) ? cc - bins : cc;
+                        const float x = data[a + c];
+                        const float y = (b >= 0) ? data[b + cc] : -0.0f;
+                        v[jj] = x + y;
+                    }
                 }
-                // keep at most a few iterations' loads in flight per wave so
-                // that the 40 staged values, not the addresses, own the VGPRs
-                if ((q & 3) == 3) __builtin_amdgcn_sched_barrier(0);
+                if (++q == kq) {
+                    q = 0;
+                    ++i;
+                }
+                // bound the loads in flight so that the staged values, not
+                // hoisted addresses, own the registers
+                if ((jj & 3) == 3) __builtin_amdgcn_sched_barrier(0);
             }
         }
         __syncthreads();
+        {
+            int i = 0, q = 0;
 #pragma unroll
-        for (int q = 0; q < MAXE; ++q) {
-            const int e = tid + q * blockDim.x;
-            if (e < total) data[e] = v[q];
+            for (int jj = 0; jj < kStage; ++jj) {
+                const int p = wave + i * NW;
+                const int c = ln + q * kWave;
+                if (p < out_rows && c < bins) data[p * bins + c] = v[jj];
+                if (++q == kq) {
+                    q = 0;
+                    ++i;
+                }
+            }
         }
         __syncthreads();
         rows = out_rows;
@@ -308,7 +331,7 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
     if (tk.kind == 0) {
         float* dst = tk.dst + ((size_t)tk.node_r0 + tk.lo) * bins;
         const int total = rows * bins;
-        for (int e = tid; e < total; e += blockDim.x)
+        for (int e = tid; e < total; e += NT)
             dst[e] = data[e];
         __syncthreads();
         return;
@@ -316,9 +339,7 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 
     // S/N of every finished row s = lo + p with s < rows_eval, one wavefront
     // per row, rows processed in lockstep rounds (uniform barrier count).
-    const int wave = tid / kWave;
-    const int nwaves = blockDim.x / kWave;
-    const int lane = tid & (kWave - 1);
+    const int nwaves = NW;
     const int s_end = min(tk.hi, tk.rows_eval);
     const int nrows = max(0, s_end - tk.lo);
     const int rounds = (nrows + nwaves - 1) / nwaves;
@@ -347,14 +368,14 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 }
 
 // One workgroup per task (the task list of one pass).
-template <int MAXE>
-__global__ __launch_bounds__(kFfaThreads) void ffa_task_kernel(const FfaTask* __restrict__ tasks,
-                                                                int ntasks, int* err)
+template <int NT>
+__global__ __launch_bounds__(NT) void ffa_task_kernel(const FfaTask* __restrict__ tasks,
+                                                       int ntasks, int* err)
 {
     extern __shared__ float lds[];
     if ((int)blockIdx.x >= ntasks) return;
     const FfaTask tk = tasks[blockIdx.x];
-    run_ffa_task<MAXE>(tk, lds, err);
+    run_ffa_task<NT>(tk, lds, err);
 }
 
 // Node i of depth d of an m-row merge tree (row order); false when that slot
@@ -457,8 +478,8 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
 // its intermediate states in a private scratch slot, so no workgroup ever
 // waits on another.  Rows written by one pass are read by the next only by
 // this workgroup, ordered by the barrier that ends every task.
-template <int MAXE>
-__global__ __launch_bounds__(kFfaThreads) void ffa_batch_kernel(
+template <int NT>
+__global__ __launch_bounds__(NT) void ffa_batch_kernel(
     const FfaTransform* __restrict__ trs, int ntr, int* __restrict__ queue, float* scratch,
     int64_t slot_floats, int* err)
 {
@@ -488,7 +509,7 @@ __global__ __launch_bounds__(kFfaThreads) void ffa_batch_kernel(
         // that the descriptor itself stays in registers
         const int32_t* depth = trs[item].depth;
         while (next_ffa_task(tr, depth, p, i, lo, buf_a, buf_b, tk))
-            run_ffa_task<MAXE>(tk, lds, err);
+            run_ffa_task<NT>(tk, lds, err);
     }
 }
 
@@ -575,37 +596,60 @@ __global__ void ffa_level_global_kernel(const float* __restrict__ src, float* __
 // Host launchers
 // ---------------------------------------------------------------------------
 
-hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipStream_t stream)
+template <int NT>
+static hipError_t launch_tasks_nt(const FfaTask* d_tasks, int ntasks, int* d_err,
+                                  hipStream_t stream, int lds)
 {
     static bool configured = false;
     if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void*)ffa_task_kernel<kFfaMaxE>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           ffa_lds_bytes());
+        hipError_t e = hipFuncSetAttribute((const void*)ffa_task_kernel<NT>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
         configured = true;
     }
+    hipLaunchKernelGGL(ffa_task_kernel<NT>, dim3(ntasks), dim3(NT), lds, stream, d_tasks, ntasks,
+                       d_err);
+    return hipGetLastError();
+}
+
+hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipStream_t stream,
+                            int shape)
+{
     if (ntasks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ffa_task_kernel<kFfaMaxE>, dim3(ntasks), dim3(kFfaThreads),
-                       ffa_lds_bytes(), stream, d_tasks, ntasks, d_err);
+    const int lds = ffa_lds_bytes(shape);
+    if (kFfaShape[shape].threads == 1024)
+        return launch_tasks_nt<1024>(d_tasks, ntasks, d_err, stream, lds);
+    return launch_tasks_nt<512>(d_tasks, ntasks, d_err, stream, lds);
+}
+
+template <int NT>
+static hipError_t launch_batch_nt(const FfaTransform* d_trs, int ntr, int* d_queue,
+                                  float* d_scratch, int64_t slot_floats, int slots, int* d_err,
+                                  hipStream_t stream, int lds)
+{
+    static bool configured = false;
+    if (!configured) {
+        hipError_t e = hipFuncSetAttribute((const void*)ffa_batch_kernel<NT>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+        if (e != hipSuccess) return e;
+        configured = true;
+    }
+    hipLaunchKernelGGL(ffa_batch_kernel<NT>, dim3(slots), dim3(NT), lds, stream, d_trs, ntr,
+                       d_queue, d_scratch, slot_floats, d_err);
     return hipGetLastError();
 }
 
 hipError_t launch_ffa_batch(const FfaTransform* d_trs, int ntr, int* d_queue, float* d_scratch,
-                            int64_t slot_floats, int slots, int* d_err, hipStream_t stream)
+                            int64_t slot_floats, int slots, int* d_err, hipStream_t stream,
+                            int shape)
 {
-    static bool configured = false;
-    if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void*)ffa_batch_kernel<kFfaMaxE>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           ffa_lds_bytes());
-        if (e != hipSuccess) return e;
-        configured = true;
-    }
     if (ntr <= 0) return hipSuccess;
-    hipLaunchKernelGGL(ffa_batch_kernel<kFfaMaxE>, dim3(slots), dim3(kFfaThreads),
-                       ffa_lds_bytes(), stream, d_trs, ntr, d_queue, d_scratch, slot_floats, d_err);
-    return hipGetLastError();
+    const int lds = ffa_lds_bytes(shape);
+    if (kFfaShape[shape].threads == 1024)
+        return launch_batch_nt<1024>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots, d_err,
+                                     stream, lds);
+    return launch_batch_nt<512>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots, d_err, stream,
+                                lds);
 }
 
 hipError_t launch_snr_rows(const float* x, int64_t rows, int cols, const int32_t* widths,

@@ -5,12 +5,11 @@
 
 namespace rt {
 
-// Dynamic LDS of the FFA kernel: gfx950 gives a workgroup up to 160 KiB; a
-// few KiB stay for the kernel's static LDS.
-static constexpr int kFfaLdsBytes = 160 * 1024 - 1024;
 static constexpr int kSegBytes = (2 << kMaxConeLevels) * 5 * 4;
 
-int ffa_lds_bytes() { return kFfaLdsBytes; }
+int ffa_lds_bytes(int shape) { return kFfaShape[shape].lds_bytes; }
+
+int ffa_default_shape() { return 0; }
 
 int ffa_height(int64_t m)
 {
@@ -19,13 +18,12 @@ int ffa_height(int64_t m)
     return h;
 }
 
-int ffa_cap_rows(int bins, int lds_bytes)
+int ffa_cap_rows(int bins, int shape)
 {
-    // data[cap*bins] + three int tables of cap entries + cone segments; also
-    // bounded by the register staging of the kernel (40 elements per thread)
-    const int avail = lds_bytes - kSegBytes;
-    int cap = avail / (4 * (bins + 3));
-    const int by_regs = (40 * kFfaThreads) / bins;
+    const int avail = kFfaShape[shape].lds_bytes - kSegBytes;
+    const int cap = avail / (4 * (bins + 3));
+    const int k = (bins + kWave - 1) / kWave;
+    const int by_regs = (kFfaShape[shape].threads / kWave) * (kStage / k);
     return std::min(cap, by_regs);
 }
 

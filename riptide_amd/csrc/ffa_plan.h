@@ -15,16 +15,22 @@ namespace rt {
 // ceil(log2(m)): number of merge levels of an m-row node (m >= 1).
 int ffa_height(int64_t m);
 
-// Rows of `bins` columns that one workgroup can stage (see ffa_lds_layout).
-int ffa_cap_rows(int bins, int lds_bytes);
+// Rows of `bins` columns that one workgroup of shape `shape` can stage: the
+// LDS layout (rows + three per-row tables + cone segments) and the register
+// staging (rows per wavefront x columns per lane <= kStage) must both fit.
+int ffa_cap_rows(int bins, int shape);
 
-// Bytes of dynamic LDS the FFA task kernel is launched with.
-int ffa_lds_bytes();
+// Bytes of dynamic LDS the FFA kernels are launched with, per shape.
+int ffa_lds_bytes(int shape);
+
+// Shape used when no tuned choice exists.
+int ffa_default_shape();
 
 struct TransformSpec {
     int64_t m = 0;          // rows
     int bins = 0;           // columns
     int cap = 0;            // rows per workgroup (ffa_cap_rows)
+    int shape = 0;          // workgroup shape the tasks will be launched with
     const float* src = nullptr;   // folded input rows
     float* buf_a = nullptr; // intermediate states (m*bins floats each)
     float* buf_b = nullptr;

@@ -5,7 +5,19 @@
 
 namespace rt {
 
-// Threads per workgroup of the FFA kernels (16 wavefronts of 64).
+// Workgroup shapes of the FFA kernels: threads and dynamic LDS bytes.  Shape
+// 0 runs one 16-wavefront workgroup per CU with ~159 KiB of LDS; shape 1 two
+// 8-wavefront workgroups per CU with ~79 KiB each.
+struct FfaShape {
+    int threads;
+    int lds_bytes;
+};
+constexpr int kFfaShapes = 2;
+constexpr FfaShape kFfaShape[kFfaShapes] = {{1024, 160 * 1024 - 1024}, {512, 80 * 1024 - 1024}};
+// Merge-level results each lane keeps in registers (rows per wavefront times
+// columns per lane must fit).
+constexpr int kStage = 48;
+// Largest workgroup of the FFA kernels.
 constexpr int kFfaThreads = 1024;
 // Wavefront width on CDNA.
 constexpr int kWave = 64;
@@ -65,7 +77,7 @@ struct FfaTransform {
     int32_t levels0;        // npass == 1: merge levels of the whole transform
     int32_t depth[kMaxPasses];
     int32_t series;         // owning series (bookkeeping)
-    int32_t pad;
+    int32_t shape;          // workgroup shape (kFfaShape index) the tiling assumes
 };
 
 // One downsampling step of one series for the packed multi-step kernel.

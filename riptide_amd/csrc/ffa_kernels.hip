@@ -22,11 +22,9 @@
 namespace rt {
 
 // LDS layout of the FFA task kernel (bytes are dynamic; see ffa_lds_bytes()):
-//   float data[cap_rows * bins]      staged rows
-//   int   tab_a[cap_rows]            per-row head (or copy) offset, in floats
-//   int   tab_b[cap_rows]            per-row tail offset, -1 for a copy
-//   int   tab_s[cap_rows]            per-row circular shift
-//   Seg   seg[2^(kMaxConeLevels+1)]  cone segments (cone tasks only)
+//   float    data[cap_rows * bins]            staged rows
+//   uint32_t tab[kFfaMaxLevels * cap_rows]    packed row map of every level
+//   Seg      seg[2^(kMaxConeLevels+1)]        cone segments (cone tasks only)
 struct Seg {
     int m;     // node rows (0: empty slot)
     int r0;    // node first row (transform-relative)
@@ -186,13 +184,45 @@ __device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int b
     *sh = (s - tt) % bins;
 }
 
+// Packed per-row table entry of one merge level: head row (10 bits), tail
+// row + 1 (10 bits, 0 = copy the head row) and circular shift (12 bits).
+__device__ __forceinline__ uint32_t pack_row(int a, int b, int sh)
+{
+    return (uint32_t)a | ((uint32_t)(b + 1) << 10) | ((uint32_t)sh << 20);
+}
+
+// Copies `rows` rows of `bins` floats from global memory into LDS, one
+// wavefront per row with the row's k column values loaded before they are
+// stored (loads in flight instead of one round trip per value).
+template <int NW>
+__device__ __forceinline__ void load_rows(float* data, const float* src, const uint32_t* grow,
+                                          int rows, int bins, int k, int wave, int lane)
+{
+    for (int p = wave; p < rows; p += NW) {
+        const float* srow = src + (size_t)__builtin_amdgcn_readfirstlane(grow[p]) * bins;
+        float* drow = data + p * bins;
+        float v[kFfaLoadK];
+#pragma unroll
+        for (int q = 0; q < kFfaLoadK; ++q) {
+            const int c = lane + q * kWave;
+            if (q < k && c < bins) v[q] = srow[c];
+        }
+#pragma unroll
+        for (int q = 0; q < kFfaLoadK; ++q) {
+            const int c = lane + q * kWave;
+            if (q < k && c < bins) drow[c] = v[q];
+        }
+        for (int c = lane + kFfaLoadK * kWave; c < bins; c += kWave) drow[c] = srow[c];
+    }
+}
+
 // Runs one FfaTask with the whole workgroup; LDS contents are dead on return.
 //
-// Every merge level is processed one row per wavefront: the row's head/tail
-// offsets and shift come from a per-row table (one broadcast LDS read each),
-// lanes walk the columns c = lane + 64*q, and each lane keeps up to kStage
-// results in registers until the barrier after which the level is written
-// back in place.
+// The row tables of all merge levels are built first (one barrier).  Every
+// level is then processed one row per wavefront: the row's packed table entry
+// is one broadcast LDS read, lanes walk the columns c = lane + 64*q, and each
+// lane keeps up to kStage results in registers until the barrier after which
+// the level is written back in place.
 template <int NT>
 __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 {
@@ -205,45 +235,68 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
     const int k = (bins + kWave - 1) / kWave;  // columns per lane
 
     float* data = lds;
-    int* tab_a = reinterpret_cast<int*>(data + (size_t)cap_rows * bins);
-    int* tab_b = tab_a + cap_rows;
-    int* tab_s = tab_b + cap_rows;
-    Seg* seg = reinterpret_cast<Seg*>(tab_s + cap_rows);
+    uint32_t* tab = reinterpret_cast<uint32_t*>(data + (size_t)cap_rows * bins);
+    Seg* seg = reinterpret_cast<Seg*>(tab + (size_t)cap_rows * kFfaMaxLevels);
     __shared__ int s_bottom;
 
-    // ---- rows needed at the bottom of the task, and where they come from
+    // ---- bottom rows and the tables of every level
     int rows;  // rows in the current level's layout
     if (tk.cone) {
         build_cone(seg, tk, &s_bottom);
         rows = (tk.levels > 0) ? s_bottom : (tk.hi - tk.lo);
-        if (rows > cap_rows) {
+        if (rows > cap_rows || tk.levels > kFfaMaxLevels) {
             // the host sizes tiles so this cannot happen; never overrun LDS
             if (tid == 0) atomicOr(err, 1);
             __syncthreads();
             return;
         }
-        // global source row of every bottom row
-        const int t = tk.levels;
+        // global source row of every bottom row (in the table slot of the
+        // deepest level, which no merge level uses), then every level's map
+        uint32_t* grow = tab + (size_t)(kFfaMaxLevels - 1) * cap_rows;
+        const int L = tk.levels;
         for (int p = tid; p < rows; p += NT) {
-            const int slot = seg_find(seg, t, p);
-            const Seg g = seg[slot];
-            tab_a[p] = (g.r0 + g.lo + (p - g.base)) * bins;
+            const Seg g = seg[seg_find(seg, L, p)];
+            grow[p] = (uint32_t)(g.r0 + g.lo + (p - g.base));
+        }
+        for (int t = 0; t < L; ++t) {
+            const int first = seg_slot(t, 0);
+            const int n = 1 << t;
+            const int rt = seg[first + n - 1].base + seg[first + n - 1].len;
+            for (int p = tid; p < rt; p += NT) {
+                int a, b, sh;
+                cone_row_map(seg, t, p, bins, &a, &b, &sh);
+                tab[(size_t)t * cap_rows + p] = pack_row(a / bins, b < 0 ? -1 : b / bins, sh);
+            }
         }
         __syncthreads();
-        for (int p = wave; p < rows; p += NW) {
-            const float* srow = tk.src + (size_t)__builtin_amdgcn_readfirstlane(tab_a[p]);
-            float* drow = data + p * bins;
-            for (int c = lane; c < bins; c += kWave) drow[c] = srow[c];
-        }
-        // the first level's table overwrites tab_a: every load must have read it
-        __syncthreads();
+        load_rows<NW>(data, tk.src, grow, rows, bins, k, wave, lane);
     } else {
         rows = tk.node_m;
+        if (tk.levels > kFfaMaxLevels) {
+            if (tid == 0) atomicOr(err, 1);
+            __syncthreads();
+            return;
+        }
+        for (int t = 0; t < tk.levels; ++t)
+            for (int p = tid; p < rows; p += NT) {
+                int a, b, sh;
+                identity_row_map(tk.node_m, t, p, bins, &a, &b, &sh);
+                tab[(size_t)t * cap_rows + p] = pack_row(a / bins, b < 0 ? -1 : b / bins, sh);
+            }
+        // contiguous rows: a flat copy with several loads in flight per lane
         const float* src = tk.src + (size_t)tk.node_r0 * bins;
         const int total = rows * bins;
-        for (int e = tid; e < total; e += NT)
-            data[e] = src[e];
+        for (int e0 = tid; e0 < total; e0 += NT * 8) {
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e0 + u * NT < total) v[u] = src[e0 + u * NT];
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (e0 + u * NT < total) data[e0 + u * NT] = v[u];
+        }
     }
+    __syncthreads();
 
     // ---- merge levels, bottom-up, in place through registers
     for (int t = tk.levels - 1; t >= 0; --t) {
@@ -251,23 +304,11 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
         if (tk.cone) {
             const int first = seg_slot(t, 0);
             const int n = 1 << t;
-            // rows of level t = sum of segment lengths (last base + len)
             out_rows = seg[first + n - 1].base + seg[first + n - 1].len;
         } else {
             out_rows = rows;
         }
-        // table of level t
-        for (int p = tid; p < out_rows; p += NT) {
-            int a, b, sh;
-            if (tk.cone)
-                cone_row_map(seg, t, p, bins, &a, &b, &sh);
-            else
-                identity_row_map(tk.node_m, t, p, bins, &a, &b, &sh);
-            tab_a[p] = a;
-            tab_b[p] = b;
-            tab_s[p] = sh;
-        }
-        __syncthreads();
+        const uint32_t* tl = tab + (size_t)t * cap_rows;
         // compute: staged value jj of this lane is row wave + NW*i, column
         // lane + 64*q, with (i, q) walking the rows of this wavefront
         float v[kStage];
@@ -285,9 +326,11 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
                     const int p = wave + i * NW;
                     valid = p < out_rows;
                     if (valid) {
-                        a = __builtin_amdgcn_readfirstlane(tab_a[p]);
-                        b = __builtin_amdgcn_readfirstlane(tab_b[p]);
-                        sh = __builtin_amdgcn_readfirstlane(tab_s[p]);
+                        const uint32_t e = __builtin_amdgcn_readfirstlane(tl[p]);
+                        a = (int)(e & 1023u) * bins;
+                        b = (int)((e >> 10) & 1023u) - 1;
+                        b = b < 0 ? -1 : b * bins;
+                        sh = (int)(e >> 20);
                     }
                 }
                 if (valid) {
@@ -337,39 +380,52 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
         return;
     }
 
-    // S/N of every finished row s = lo + p with s < rows_eval, one wavefront
-    // per row, rows processed in lockstep rounds (uniform barrier count).
-    const int nwaves = NW;
+    // S/N of every finished row s = lo + p with s < rows_eval, in two
+    // thread-level phases (no cross-lane traffic):
+    //  1. one thread per row turns the row into its circular prefix sums in
+    //     place, accumulating left to right in float-float (two-sum), so
+    //     every partial sum is the float32 rounding of the exact sum, as the
+    //     reference's float64 accumulator gives (kernels.hpp:73-90);
+    //  2. one thread per (row, width) scans the row for the largest window
+    //     sum and writes that width's S/N (snr.hpp:44-54).
     const int s_end = min(tk.hi, tk.rows_eval);
     const int nrows = max(0, s_end - tk.lo);
-    const int rounds = (nrows + nwaves - 1) / nwaves;
-    for (int r = 0; r < rounds; ++r) {
-        const int p = r * nwaves + wave;
-        const bool active = p < nrows;
-        float* row = data + (size_t)(active ? p : 0) * bins;
-        float* out = tk.snr + (size_t)(tk.lo + p) * tk.nw;
-        if (bins <= kWave * kSnrFfaK) {
-            float own[kSnrFfaK];
-            if (active)
-                snr_prefix_inplace<kSnrFfaK>(row, bins, lane, own);
-            __syncthreads();
-            if (active)
-                snr_widths<kSnrFfaK>(row, bins, lane, own, tk.widths, tk.hb, tk.nw, tk.stdnoise,
-                                     out);
-        } else {
-            if (active)
-                snr_prefix_long(row, row, bins, lane);
-            __syncthreads();
-            if (active)
-                snr_widths_long(row, bins, lane, tk.widths, tk.hb, tk.nw, tk.stdnoise, out);
+    for (int p = tid; p < nrows; p += NT) {
+        float* row = data + (size_t)p * bins;
+        float sh = 0.0f, sl = 0.0f;
+        for (int i = 0; i < bins; ++i) {
+            float s, e;
+            two_sum(sh, row[i], s, e);
+            sh = s;
+            sl += e;
+            const float r = sh + sl;  // rounded partial sum
+            sl -= r - sh;             // keep (sh, sl) exact: sh + sl unchanged
+            sh = r;
+            row[i] = r;
         }
-        __syncthreads();
     }
+    __syncthreads();
+    const int nw = tk.nw;
+    for (int job = tid; job < nrows * nw; job += NT) {
+        const int p = job / nw;
+        const int iw = job - p * nw;
+        const float* cps = data + (size_t)p * bins;
+        const int w = tk.widths[iw];
+        const float total = cps[bins - 1];
+        // i + w < bins: plain differences; beyond: the extension cps[j] + total
+        float dm = -INFINITY;
+        const int split = bins - w;
+        for (int i = 0; i < split; ++i) dm = fmaxf(dm, cps[i + w] - cps[i]);
+        for (int i = split; i < bins; ++i) dm = fmaxf(dm, (cps[i + w - bins] + total) - cps[i]);
+        const float snr = __fmaf_rn(dm, tk.hb[2 * iw], -(tk.hb[2 * iw + 1] * total)) / tk.stdnoise;
+        tk.snr[(size_t)(tk.lo + p) * nw + iw] = snr;
+    }
+    __syncthreads();
 }
 
 // One workgroup per task (the task list of one pass).
 template <int NT>
-__global__ __launch_bounds__(NT) void ffa_task_kernel(const FfaTask* __restrict__ tasks,
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void ffa_task_kernel(const FfaTask* __restrict__ tasks,
                                                        int ntasks, int* err)
 {
     extern __shared__ float lds[];
@@ -479,7 +535,7 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
 // waits on another.  Rows written by one pass are read by the next only by
 // this workgroup, ordered by the barrier that ends every task.
 template <int NT>
-__global__ __launch_bounds__(NT) void ffa_batch_kernel(
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void ffa_batch_kernel(
     const FfaTransform* __restrict__ trs, int ntr, int* __restrict__ queue, float* scratch,
     int64_t slot_floats, int* err)
 {

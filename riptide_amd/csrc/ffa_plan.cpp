@@ -20,11 +20,13 @@ int ffa_height(int64_t m)
 
 int ffa_cap_rows(int bins, int shape)
 {
+    // LDS: rows of `bins` floats plus one packed table entry per row and level
     const int avail = kFfaShape[shape].lds_bytes - kSegBytes;
-    const int cap = avail / (4 * (bins + 3));
+    const int cap = avail / (4 * (bins + kFfaMaxLevels));
     const int k = (bins + kWave - 1) / kWave;
+    if (k > kStage || bins >= 4096) return 0;
     const int by_regs = (kFfaShape[shape].threads / kWave) * (kStage / k);
-    return std::min(cap, by_regs);
+    return std::min(std::min(cap, by_regs), kFfaMaxCap);
 }
 
 // Nodes of the merge tree at depth d, in row order (block.hpp:30-43 split).

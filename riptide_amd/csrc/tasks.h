@@ -17,6 +17,12 @@ constexpr FfaShape kFfaShape[kFfaShapes] = {{1024, 160 * 1024 - 1024}, {512, 80 
 // Merge-level results each lane keeps in registers (rows per wavefront times
 // columns per lane must fit).
 constexpr int kStage = 48;
+// Merge levels whose row tables one task keeps in LDS (all at once).
+constexpr int kFfaMaxLevels = 12;
+// Columns per lane staged at once by the row loads.
+constexpr int kFfaLoadK = 9;
+// Largest rows per workgroup (the packed row tables hold 10-bit rows).
+constexpr int kFfaMaxCap = 1023;
 // Largest workgroup of the FFA kernels.
 constexpr int kFfaThreads = 1024;
 // Wavefront width on CDNA.

@@ -95,6 +95,48 @@ def cpu_baseline(nsamp, tsamp):
     }
 
 
+def fwd_bwd_timing(torch, _lib, reps=5):
+    """One float32 series of 2^20 samples at 256 us (white noise + Gaussian
+    pulses, P = 1.3711 s, FWHM 2 %, S/N 20), normalise -> periodogram over
+    0.5-2 s with 240-260 bins; loss = sum over widths of the max S/N within
+    +-0.5 % of the injected period.  Returns forward and forward + backward
+    milliseconds (HIP events, median of `reps`)."""
+    from riptide_amd.autograd import periodogram_fn, periodogram_grid
+    n, tsamp, p0 = 2 ** 20, 256e-6, 1.3711
+    t = np.arange(n) * tsamp
+    ph = (t / p0) % 1.0
+    sig = np.exp(-0.5 * ((ph - 0.5) / (0.02 / 2.355)) ** 2)
+    sig *= 20.0 / np.sqrt((sig ** 2).sum())
+    x = (np.random.default_rng(4).standard_normal(n) + sig).astype(np.float32)
+    P, _ = periodogram_grid(n, tsamp, 0.5, 2.0, 240, 260)
+    dev = _lib.device()
+    mask = torch.as_tensor(np.abs(P / p0 - 1) <= 0.005, device=dev)
+    xg = torch.tensor(x, device=dev, requires_grad=True)
+
+    def fwd():
+        snrs = periodogram_fn(xg, tsamp, None, 0.5, 2.0, 240, 260)
+        return snrs[mask].max(dim=0).values.sum()
+
+    def run(backward):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        xg.grad = None
+        e0.record()
+        loss = fwd()
+        if backward:
+            loss.backward()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    run(True)
+    f = sorted(run(False) for _ in range(reps))[reps // 2]
+    fb = sorted(run(True) for _ in range(reps))[reps // 2]
+    return {"config": "2^20 samples, tsamp 256 us, P 0.5-2 s, bins 240-260, loss = sum_w max "
+                      "S/N within +-0.5% of 1.3711 s",
+            "fwd_ms": round(f, 3), "fwd_bwd_ms": round(fb, 3), "ratio": round(fb / f, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,6 +248,10 @@ def main():
         plain_tps = nplain / plain_s
         log(f"plain: {nplain} trials in {plain_s:.3f} s -> {plain_tps:.1f} trials/s")
 
+    # differentiable search (config 4 shape): forward vs forward + adjoint
+    grad = fwd_bwd_timing(torch, _lib)
+    log(f"fwd+bwd: {grad}")
+
     # roofline of the dominant kernel: persistent FFA + S/N kernel
     snr_bytes = sum(batch.lengths[i] * batch.widths[i].size * 4 for i in range(args.trials))
     fold_bytes = st["fold_floats"] * 4
@@ -264,6 +310,7 @@ def main():
                     "frac": round(lds_achieved / LDS_PEAK_GBS, 4),
                     "bytes": "8 B per FFA float add"},
         },
+        "fwd_bwd": grad,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

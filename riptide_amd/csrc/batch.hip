@@ -340,9 +340,8 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
 
     int cus = 0;
     BT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-    // workgroups resident at once: one per CU for the 1024-thread shape, two
-    // for the 512-thread shape
-    b->slots = std::max(1, cus) * (kFfaShape[b->shape].threads == 1024 ? 1 : 2);
+    // workgroups resident at once: as many per CU as the LDS of the shape allows
+    b->slots = std::max(1, cus) * std::max(1, (160 * 1024) / (kFfaShape[b->shape].lds_bytes + 1024));
     b->ds_floats = max_ds;
     b->prep_floats = (deredden || normalise) ? max_prep : 1;
     b->slot_floats = 2 * max_fold;

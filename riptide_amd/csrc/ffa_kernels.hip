@@ -22,6 +22,7 @@
 namespace rt {
 
 // LDS layout of the FFA task kernel (bytes are dynamic; see ffa_lds_bytes()):
+//   float    zero[bins]                       all -0.0f (tail of a copied row)
 //   float    data[cap_rows * bins]            staged rows
 //   uint32_t tab[kFfaMaxLevels * cap_rows]    packed row map of every level
 //   Seg      seg[2^(kMaxConeLevels+1)]        cone segments (cone tasks only)
@@ -216,6 +217,114 @@ __device__ __forceinline__ void load_rows(float* data, const float* src, const u
     }
 }
 
+// Rows in the layout of merge level t of a task (the rows that level writes).
+__device__ __forceinline__ int level_rows(bool cone, const Seg* seg, int t, int rows)
+{
+    if (!cone) return rows;
+    const int first = seg_slot(t, 0);
+    const int n = 1 << t;
+    return seg[first + n - 1].base + seg[first + n - 1].len;
+}
+
+// All merge levels of a task, bottom-up, in place, with at most K columns per
+// lane (bins <= 64*K; exactly K when EXACT) known at compile time.
+// Wavefront w owns rows w, w+NW, ...; lane l owns columns l + 64*q.  Per
+// element: two LDS reads (head column c, tail column (c + shift) mod bins,
+// the wrap chosen by one compare against a per-row scalar) and one add; the
+// results stay in registers until the barrier, then overwrite the level in
+// place.  Row -1 of `data` is the -0.0f row, the tail of copied rows.
+// Kept out of line: one register allocation per column count.
+template <int NW, int K, int STAGE, bool EXACT>
+__device__ __forceinline__ int merge_levels_k(const Seg* seg, float* data,
+                                                        const uint32_t* tab, int rows, int levels,
+                                                        bool cone, int bins, int cap_rows,
+                                                        int wave, int lane)
+{
+    constexpr int R = ffa_rows_per_wave(K, STAGE);
+    static_assert(R >= 1, "column count beyond the register staging");
+    const int kd = (bins + kWave - 1) / kWave;
+    const bool last_ok = lane + (K - 1) * kWave < bins;
+    for (int t = levels - 1; t >= 0; --t) {
+        const int out_rows = level_rows(cone, seg, t, rows);
+        const uint32_t* tl = tab + (size_t)t * cap_rows;
+        const int nr = (out_rows - wave + NW - 1) / NW;
+        // this wavefront's row-table entries, all read before the first use
+        uint32_t ent[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) ent[i] = tl[min(wave + i * NW, out_rows - 1)];
+        float v[R * K];
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            if (i < nr) {
+                const uint32_t e = __builtin_amdgcn_readfirstlane(ent[i]);
+                const int a = (int)(e & 1023u);
+                const int bt = (int)((e >> 10) & 1023u) - 1;
+                const int sh = (int)(e >> 20);
+                const float* hrow = data + a * bins + lane;
+                // tail column c + sh before and after the wrap
+                const float* trow = data + bt * bins + lane + sh;
+                const float* twrap = trow - bins;
+                const int lim = bins - sh;
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    if (EXACT || q < kd) {
+                        const float* tp = lane + q * kWave >= lim ? twrap : trow;
+                        v[i * K + q] = hrow[q * kWave] + tp[q * kWave];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < R; ++i) {
+            if (i < nr) {
+                float* orow = data + (wave + i * NW) * bins + lane;
+#pragma unroll
+                for (int q = 0; q < K; ++q) {
+                    if (EXACT) {
+                        if (q < K - 1 || last_ok) orow[q * kWave] = v[i * K + q];
+                    } else if (q < kd && lane + q * kWave < bins) {
+                        orow[q * kWave] = v[i * K + q];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        rows = out_rows;
+    }
+    return rows;
+}
+
+// Dispatch on the columns per lane: exact instantiations up to 16 columns
+// (bins <= 1024), then one per rows-per-wavefront class.
+template <int NW, int STAGE>
+__device__ __forceinline__ int merge_levels(const Seg* seg, float* data, const uint32_t* tab,
+                                            int rows, int levels, bool cone, int bins,
+                                            int cap_rows, int wave, int lane)
+{
+    const int k = (bins + kWave - 1) / kWave;
+#define RT_MERGE_ARGS seg, data, tab, rows, levels, cone, bins, cap_rows, wave, lane
+    switch (k) {
+#define RT_MERGE_K(K)                                                                          \
+    case K:                                                                                    \
+        return merge_levels_k<NW, K, STAGE, true>(RT_MERGE_ARGS);
+    RT_MERGE_K(1) RT_MERGE_K(2) RT_MERGE_K(3) RT_MERGE_K(4) RT_MERGE_K(5) RT_MERGE_K(6)
+    RT_MERGE_K(7) RT_MERGE_K(8) RT_MERGE_K(9) RT_MERGE_K(10) RT_MERGE_K(11) RT_MERGE_K(12)
+    RT_MERGE_K(13) RT_MERGE_K(14) RT_MERGE_K(15) RT_MERGE_K(16)
+#undef RT_MERGE_K
+    default: break;
+    }
+    // wide rows: R rows per wavefront of up to STAGE/R columns
+    if (STAGE / 4 > 16 && k <= STAGE / 4)
+        return merge_levels_k<NW, (STAGE / 4 > 16 ? STAGE / 4 : 16), STAGE, false>(RT_MERGE_ARGS);
+    if (STAGE / 3 > 16 && k <= STAGE / 3)
+        return merge_levels_k<NW, (STAGE / 3 > 16 ? STAGE / 3 : 16), STAGE, false>(RT_MERGE_ARGS);
+    if (STAGE / 2 > 16 && k <= STAGE / 2)
+        return merge_levels_k<NW, (STAGE / 2 > 16 ? STAGE / 2 : 16), STAGE, false>(RT_MERGE_ARGS);
+    return merge_levels_k<NW, STAGE, STAGE, false>(RT_MERGE_ARGS);
+#undef RT_MERGE_ARGS
+}
+
 // Runs one FfaTask with the whole workgroup; LDS contents are dead on return.
 //
 // The row tables of all merge levels are built first (one barrier).  Every
@@ -223,7 +332,7 @@ __device__ __forceinline__ void load_rows(float* data, const float* src, const u
 // is one broadcast LDS read, lanes walk the columns c = lane + 64*q, and each
 // lane keeps up to kStage results in registers until the barrier after which
 // the level is written back in place.
-template <int NT>
+template <int NT, int STAGE>
 __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 {
     constexpr int NW = NT / kWave;
@@ -234,10 +343,13 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
     const int cap_rows = tk.pad;
     const int k = (bins + kWave - 1) / kWave;  // columns per lane
 
-    float* data = lds;
+    // row -1 of data is all -0.0f; rows 0..cap_rows-1 hold the staged rows
+    float* data = lds + bins;
     uint32_t* tab = reinterpret_cast<uint32_t*>(data + (size_t)cap_rows * bins);
     Seg* seg = reinterpret_cast<Seg*>(tab + (size_t)cap_rows * kFfaMaxLevels);
     __shared__ int s_bottom;
+
+    for (int c = tid; c < bins; c += NT) lds[c] = -0.0f;
 
     // ---- bottom rows and the tables of every level
     int rows;  // rows in the current level's layout
@@ -299,76 +411,8 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
     __syncthreads();
 
     // ---- merge levels, bottom-up, in place through registers
-    for (int t = tk.levels - 1; t >= 0; --t) {
-        int out_rows;
-        if (tk.cone) {
-            const int first = seg_slot(t, 0);
-            const int n = 1 << t;
-            out_rows = seg[first + n - 1].base + seg[first + n - 1].len;
-        } else {
-            out_rows = rows;
-        }
-        const uint32_t* tl = tab + (size_t)t * cap_rows;
-        // compute: staged value jj of this lane is row wave + NW*i, column
-        // lane + 64*q, with (i, q) walking the rows of this wavefront
-        float v[kStage];
-        // the (row, column) walk below is the same on every level: hide lane
-        // and k from the optimiser so that it is not precomputed for all
-        // kStage steps outside the level loop
-        int ln = lane, kq = k;
-        asm volatile("" : "+v"(ln), "+s"(kq));
-        {
-            int i = 0, q = 0, a = 0, b = -1, sh = 0;
-            bool valid = false;
-#pragma unroll
-            for (int jj = 0; jj < kStage; ++jj) {
-                if (q == 0) {
-                    const int p = wave + i * NW;
-                    valid = p < out_rows;
-                    if (valid) {
-                        const uint32_t e = __builtin_amdgcn_readfirstlane(tl[p]);
-                        a = (int)(e & 1023u) * bins;
-                        b = (int)((e >> 10) & 1023u) - 1;
-                        b = b < 0 ? -1 : b * bins;
-                        sh = (int)(e >> 20);
-                    }
-                }
-                if (valid) {
-                    const int c = ln + q * kWave;
-                    if (c < bins) {
-                        int cc = c + sh;
-                        cc = (cc >= bins) ? cc - bins : cc;
-                        const float x = data[a + c];
-                        const float y = (b >= 0) ? data[b + cc] : -0.0f;
-                        v[jj] = x + y;
-                    }
-                }
-                if (++q == kq) {
-                    q = 0;
-                    ++i;
-                }
-                // bound the loads in flight so that the staged values, not
-                // hoisted addresses, own the registers
-                if ((jj & 3) == 3) __builtin_amdgcn_sched_barrier(0);
-            }
-        }
-        __syncthreads();
-        {
-            int i = 0, q = 0;
-#pragma unroll
-            for (int jj = 0; jj < kStage; ++jj) {
-                const int p = wave + i * NW;
-                const int c = ln + q * kWave;
-                if (p < out_rows && c < bins) data[p * bins + c] = v[jj];
-                if (++q == kq) {
-                    q = 0;
-                    ++i;
-                }
-            }
-        }
-        __syncthreads();
-        rows = out_rows;
-    }
+    rows = merge_levels<NW, STAGE>(seg, data, tab, rows, tk.levels, tk.cone != 0, bins, cap_rows,
+                                   wave, lane);
 
     // ---- outputs
     if (tk.kind == 0) {
@@ -424,14 +468,14 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 }
 
 // One workgroup per task (the task list of one pass).
-template <int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void ffa_task_kernel(const FfaTask* __restrict__ tasks,
-                                                       int ntasks, int* err)
+template <int NT, int STAGE, int WPS>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) void ffa_task_kernel(
+    const FfaTask* __restrict__ tasks, int ntasks, int* err)
 {
     extern __shared__ float lds[];
     if ((int)blockIdx.x >= ntasks) return;
     const FfaTask tk = tasks[blockIdx.x];
-    run_ffa_task<NT>(tk, lds, err);
+    run_ffa_task<NT, STAGE>(tk, lds, err);
 }
 
 // Node i of depth d of an m-row merge tree (row order); false when that slot
@@ -534,8 +578,8 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
 // its intermediate states in a private scratch slot, so no workgroup ever
 // waits on another.  Rows written by one pass are read by the next only by
 // this workgroup, ordered by the barrier that ends every task.
-template <int NT>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void ffa_batch_kernel(
+template <int NT, int STAGE, int WPS>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) void ffa_batch_kernel(
     const FfaTransform* __restrict__ trs, int ntr, int* __restrict__ queue, float* scratch,
     int64_t slot_floats, int* err)
 {
@@ -565,7 +609,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4, 4))) void
         // that the descriptor itself stays in registers
         const int32_t* depth = trs[item].depth;
         while (next_ffa_task(tr, depth, p, i, lo, buf_a, buf_b, tk))
-            run_ffa_task<NT>(tk, lds, err);
+            run_ffa_task<NT, STAGE>(tk, lds, err);
     }
 }
 
@@ -652,19 +696,19 @@ __global__ void ffa_level_global_kernel(const float* __restrict__ src, float* __
 // Host launchers
 // ---------------------------------------------------------------------------
 
-template <int NT>
-static hipError_t launch_tasks_nt(const FfaTask* d_tasks, int ntasks, int* d_err,
-                                  hipStream_t stream, int lds)
+template <int NT, int STAGE, int WPS>
+static hipError_t launch_tasks_shape(const FfaTask* d_tasks, int ntasks, int* d_err,
+                                     hipStream_t stream, int lds)
 {
     static bool configured = false;
     if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void*)ffa_task_kernel<NT>,
+        hipError_t e = hipFuncSetAttribute((const void*)ffa_task_kernel<NT, STAGE, WPS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL(ffa_task_kernel<NT>, dim3(ntasks), dim3(NT), lds, stream, d_tasks, ntasks,
-                       d_err);
+    hipLaunchKernelGGL((ffa_task_kernel<NT, STAGE, WPS>), dim3(ntasks), dim3(NT), lds, stream,
+                       d_tasks, ntasks, d_err);
     return hipGetLastError();
 }
 
@@ -673,25 +717,27 @@ hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipS
 {
     if (ntasks <= 0) return hipSuccess;
     const int lds = ffa_lds_bytes(shape);
-    if (kFfaShape[shape].threads == 1024)
-        return launch_tasks_nt<1024>(d_tasks, ntasks, d_err, stream, lds);
-    return launch_tasks_nt<512>(d_tasks, ntasks, d_err, stream, lds);
+    switch (shape) {
+    case 0: return launch_tasks_shape<1024, 40, 4>(d_tasks, ntasks, d_err, stream, lds);
+    case 1: return launch_tasks_shape<512, 40, 4>(d_tasks, ntasks, d_err, stream, lds);
+    default: return launch_tasks_shape<512, 72, 2>(d_tasks, ntasks, d_err, stream, lds);
+    }
 }
 
-template <int NT>
-static hipError_t launch_batch_nt(const FfaTransform* d_trs, int ntr, int* d_queue,
-                                  float* d_scratch, int64_t slot_floats, int slots, int* d_err,
-                                  hipStream_t stream, int lds)
+template <int NT, int STAGE, int WPS>
+static hipError_t launch_batch_shape(const FfaTransform* d_trs, int ntr, int* d_queue,
+                                     float* d_scratch, int64_t slot_floats, int slots, int* d_err,
+                                     hipStream_t stream, int lds)
 {
     static bool configured = false;
     if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void*)ffa_batch_kernel<NT>,
+        hipError_t e = hipFuncSetAttribute((const void*)ffa_batch_kernel<NT, STAGE, WPS>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, lds);
         if (e != hipSuccess) return e;
         configured = true;
     }
-    hipLaunchKernelGGL(ffa_batch_kernel<NT>, dim3(slots), dim3(NT), lds, stream, d_trs, ntr,
-                       d_queue, d_scratch, slot_floats, d_err);
+    hipLaunchKernelGGL((ffa_batch_kernel<NT, STAGE, WPS>), dim3(slots), dim3(NT), lds, stream,
+                       d_trs, ntr, d_queue, d_scratch, slot_floats, d_err);
     return hipGetLastError();
 }
 
@@ -701,11 +747,17 @@ hipError_t launch_ffa_batch(const FfaTransform* d_trs, int ntr, int* d_queue, fl
 {
     if (ntr <= 0) return hipSuccess;
     const int lds = ffa_lds_bytes(shape);
-    if (kFfaShape[shape].threads == 1024)
-        return launch_batch_nt<1024>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots, d_err,
-                                     stream, lds);
-    return launch_batch_nt<512>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots, d_err, stream,
-                                lds);
+    switch (shape) {
+    case 0:
+        return launch_batch_shape<1024, 40, 4>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots,
+                                               d_err, stream, lds);
+    case 1:
+        return launch_batch_shape<512, 40, 4>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots,
+                                              d_err, stream, lds);
+    default:
+        return launch_batch_shape<512, 72, 2>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots,
+                                              d_err, stream, lds);
+    }
 }
 
 hipError_t launch_snr_rows(const float* x, int64_t rows, int cols, const int32_t* widths,

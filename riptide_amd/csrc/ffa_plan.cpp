@@ -22,10 +22,12 @@ int ffa_cap_rows(int bins, int shape)
 {
     // LDS: rows of `bins` floats plus one packed table entry per row and level
     const int avail = kFfaShape[shape].lds_bytes - kSegBytes;
-    const int cap = avail / (4 * (bins + kFfaMaxLevels));
+    // (one more row holds the -0.0f row the copy rows add)
+    const int cap = (avail - 4 * bins) / (4 * (bins + kFfaMaxLevels));
     const int k = (bins + kWave - 1) / kWave;
-    if (k > kStage || bins >= 4096) return 0;
-    const int by_regs = (kFfaShape[shape].threads / kWave) * (kStage / k);
+    const int stage = kFfaShape[shape].stage;
+    if (k > stage || bins >= 4096) return 0;
+    const int by_regs = (kFfaShape[shape].threads / kWave) * ffa_rows_per_wave(k, stage);
     return std::min(std::min(cap, by_regs), kFfaMaxCap);
 }
 

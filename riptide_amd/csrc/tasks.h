@@ -8,15 +8,30 @@ namespace rt {
 // Workgroup shapes of the FFA kernels: threads and dynamic LDS bytes.  Shape
 // 0 runs one 16-wavefront workgroup per CU with ~159 KiB of LDS; shape 1 two
 // 8-wavefront workgroups per CU with ~79 KiB each.
+// `stage` is the number of merge-level results each lane keeps in registers
+// (rows per wavefront times columns per lane must fit), `waves_per_simd` the
+// occupancy the kernel is compiled for (4: 128 VGPRs, 2: 256 VGPRs).
 struct FfaShape {
     int threads;
     int lds_bytes;
+    int stage;
+    int waves_per_simd;
 };
-constexpr int kFfaShapes = 2;
-constexpr FfaShape kFfaShape[kFfaShapes] = {{1024, 160 * 1024 - 1024}, {512, 80 * 1024 - 1024}};
-// Merge-level results each lane keeps in registers (rows per wavefront times
-// columns per lane must fit).
-constexpr int kStage = 48;
+constexpr int kFfaShapes = 3;
+constexpr FfaShape kFfaShape[kFfaShapes] = {
+    {1024, 160 * 1024 - 1024, 40, 4},  // one 16-wave workgroup per CU
+    {512, 80 * 1024 - 1024, 40, 4},    // two 8-wave workgroups per CU
+    {512, 160 * 1024 - 1024, 72, 2},   // one 8-wave workgroup per CU, deep staging
+};
+// Largest stage over the shapes.
+constexpr int kStage = 72;
+// Rows per wavefront per merge level with k columns per lane: what the
+// staging holds, at most 16 (narrow rows would otherwise need more row-table
+// registers than values).
+constexpr int ffa_rows_per_wave(int k, int stage)
+{
+    return (stage / k) < 16 ? (stage / k) : 16;
+}
 // Merge levels whose row tables one task keeps in LDS (all at once).
 constexpr int kFfaMaxLevels = 12;
 // Columns per lane staged at once by the row loads.

@@ -35,6 +35,22 @@ def test_ffa2_bit_exact(lc, oracle, shape):
     assert np.array_equal(got, want), f"max |diff| {np.abs(got - want).max()}"
 
 
+@pytest.mark.parametrize("wg_shape", [0, 1, 2])
+@pytest.mark.parametrize("shape", [(150, 260), (2684, 260), (1342, 520), (97, 1000)])
+def test_ffa2_workgroup_shapes(lc, oracle, wg_shape, shape):
+    """Every workgroup shape (threads, LDS budget, register staging) tiles the
+    same transform bit-exactly."""
+    from riptide_amd import _lib
+    rng = np.random.default_rng(shape[0] + wg_shape)
+    x = rng.standard_normal(shape).astype(np.float32)
+    _lib.call("rt_set_ffa_shape", wg_shape)
+    try:
+        got = lc.ffa2(x)
+    finally:
+        _lib.call("rt_set_ffa_shape", -1)
+    assert np.array_equal(got, oracle.oracle().ffa2(x))
+
+
 def test_ffa2_kat_8x8(lc):
     x = np.zeros((8, 8), np.float32)
     x[:, 7] = 1

@@ -19,6 +19,16 @@ __device__ __forceinline__ int merge_row(float k, int s)
     return (int)__fmaf_rn(k, (float)s, 0.5f);
 }
 
+// Global-memory views of plain pointers: loads and stores through them are
+// global_* instructions rather than flat ones (flat accesses also count
+// against the LDS wait counter).
+typedef __attribute__((address_space(1))) const float gcfloat;
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) const int32_t gcint;
+__device__ __forceinline__ gcfloat* as_global(const float* p) { return (gcfloat*)p; }
+__device__ __forceinline__ gfloat* as_global(float* p) { return (gfloat*)p; }
+__device__ __forceinline__ gcint* as_global(const int32_t* p) { return (gcint*)p; }
+
 // Error-free sum: s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(float a, float b, float& s, float& e)
 {
@@ -36,6 +46,27 @@ __device__ __forceinline__ void ff_add(float ah, float al, float bh, float bl, f
     e += al + bl;
     rh = s + e;
     rl = e - (rh - s);
+}
+
+// Maximum over the 64 lanes of a wavefront with DPP moves (no LDS traffic),
+// returned to every lane.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dpp_max(float v)
+{
+    const int o = __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROW_MASK,
+                                              0xF, false);
+    return fmaxf(v, __int_as_float(o));
+}
+
+__device__ __forceinline__ float wave_max_all(float v)
+{
+    v = dpp_max<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    v = dpp_max<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    v = dpp_max<0x141, 0xF>(v);  // row_half_mirror: 8-lane groups
+    v = dpp_max<0x140, 0xF>(v);  // row_mirror: 16-lane rows
+    v = dpp_max<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_max<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), kWave - 1));
 }
 
 __device__ __forceinline__ float wave_max(float v)

@@ -59,7 +59,7 @@ __device__ __forceinline__ int seg_find(const Seg* seg, int t, int p)
 // Builds the cone of a tile top-down: level 0 is the tile itself; level t+1
 // holds the rows of the children that level t reads.  Returns R_levels (rows
 // at the bottom) in *bottom_rows.  Must be called by the whole workgroup.
-__device__ void build_cone(Seg* seg, const FfaTask& tk, int* shared_int)
+__device__ __forceinline__ void build_cone(Seg* seg, const FfaTask& tk, int* shared_int)
 {
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -126,7 +126,8 @@ __device__ void build_cone(Seg* seg, const FfaTask& tk, int* shared_int)
 }
 
 // Row map of one merge level of a whole-node (identity layout) task: row p of
-// the node at relative depth t.
+// the node at relative depth t reads head row *a and tail row *b (-1: none)
+// rolled left by *sh.
 __device__ __forceinline__ void identity_row_map(int node_m, int t, int p, int bins, int* a,
                                                  int* b, int* sh)
 {
@@ -141,7 +142,7 @@ __device__ __forceinline__ void identity_row_map(int node_m, int t, int p, int b
         }
     }
     if (m <= 1) {
-        *a = p * bins;
+        *a = p;
         *b = -1;
         *sh = 0;
         return;
@@ -153,8 +154,8 @@ __device__ __forceinline__ void identity_row_map(int node_m, int t, int p, int b
     const int s = p - r0;
     const int h = merge_row(kh, s);
     const int tt = merge_row(kt, s);
-    *a = (r0 + h) * bins;
-    *b = (r0 + mh + tt) * bins;
+    *a = r0 + h;
+    *b = r0 + mh + tt;
     *sh = (s - tt) % bins;
 }
 
@@ -169,7 +170,7 @@ __device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int b
     const Seg cb = seg[seg_slot(t + 1, 2 * j + 1)];
     const int s = g.lo + (p - g.base);
     if (g.m == 1) {
-        *a = (ca.base + (s - ca.lo)) * bins;
+        *a = ca.base + (s - ca.lo);
         *b = -1;
         *sh = 0;
         return;
@@ -180,8 +181,8 @@ __device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int b
     const float kt = ((float)mt - 1.0f) / ((float)g.m - 1.0f);
     const int h = merge_row(kh, s);
     const int tt = merge_row(kt, s);
-    *a = (ca.base + (h - ca.lo)) * bins;
-    *b = (cb.base + (tt - cb.lo)) * bins;
+    *a = ca.base + (h - ca.lo);
+    *b = cb.base + (tt - cb.lo);
     *sh = (s - tt) % bins;
 }
 
@@ -190,31 +191,6 @@ __device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int b
 __device__ __forceinline__ uint32_t pack_row(int a, int b, int sh)
 {
     return (uint32_t)a | ((uint32_t)(b + 1) << 10) | ((uint32_t)sh << 20);
-}
-
-// Copies `rows` rows of `bins` floats from global memory into LDS, one
-// wavefront per row with the row's k column values loaded before they are
-// stored (loads in flight instead of one round trip per value).
-template <int NW>
-__device__ __forceinline__ void load_rows(float* data, const float* src, const uint32_t* grow,
-                                          int rows, int bins, int k, int wave, int lane)
-{
-    for (int p = wave; p < rows; p += NW) {
-        const float* srow = src + (size_t)__builtin_amdgcn_readfirstlane(grow[p]) * bins;
-        float* drow = data + p * bins;
-        float v[kFfaLoadK];
-#pragma unroll
-        for (int q = 0; q < kFfaLoadK; ++q) {
-            const int c = lane + q * kWave;
-            if (q < k && c < bins) v[q] = srow[c];
-        }
-#pragma unroll
-        for (int q = 0; q < kFfaLoadK; ++q) {
-            const int c = lane + q * kWave;
-            if (q < k && c < bins) drow[c] = v[q];
-        }
-        for (int c = lane + kFfaLoadK * kWave; c < bins; c += kWave) drow[c] = srow[c];
-    }
 }
 
 // Rows in the layout of merge level t of a task (the rows that level writes).
@@ -226,126 +202,197 @@ __device__ __forceinline__ int level_rows(bool cone, const Seg* seg, int t, int 
     return seg[first + n - 1].base + seg[first + n - 1].len;
 }
 
-// All merge levels of a task, bottom-up, in place, with at most K columns per
-// lane (bins <= 64*K; exactly K when EXACT) known at compile time.
-// Wavefront w owns rows w, w+NW, ...; lane l owns columns l + 64*q.  Per
-// element: two LDS reads (head column c, tail column (c + shift) mod bins,
-// the wrap chosen by one compare against a per-row scalar) and one add; the
-// results stay in registers until the barrier, then overwrite the level in
-// place.  Row -1 of `data` is the -0.0f row, the tail of copied rows.
-// Kept out of line: one register allocation per column count.
+// Everything of a task that depends on the row length, with at most K
+// columns per lane (bins <= 64*K; exactly K when EXACT) known at compile time.
+// LDS rows are `pitch` floats apart (odd, so that lanes reading the same
+// column of consecutive rows hit distinct banks).  Wavefront w owns rows
+// w, w+NW, ...; lane l owns columns l + 64*q.
 template <int NW, int K, int STAGE, bool EXACT>
-__device__ __forceinline__ int merge_levels_k(const Seg* seg, float* data,
-                                                        const uint32_t* tab, int rows, int levels,
-                                                        bool cone, int bins, int cap_rows,
-                                                        int wave, int lane)
-{
-    constexpr int R = ffa_rows_per_wave(K, STAGE);
+struct RowCore {
+    static constexpr int R = ffa_rows_per_wave(K, STAGE);
     static_assert(R >= 1, "column count beyond the register staging");
-    const int kd = (bins + kWave - 1) / kWave;
-    const bool last_ok = lane + (K - 1) * kWave < bins;
-    for (int t = levels - 1; t >= 0; --t) {
-        const int out_rows = level_rows(cone, seg, t, rows);
-        const uint32_t* tl = tab + (size_t)t * cap_rows;
-        const int nr = (out_rows - wave + NW - 1) / NW;
-        // this wavefront's row-table entries, all read before the first use
-        uint32_t ent[R];
-#pragma unroll
-        for (int i = 0; i < R; ++i) ent[i] = tl[min(wave + i * NW, out_rows - 1)];
-        float v[R * K];
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            if (i < nr) {
-                const uint32_t e = __builtin_amdgcn_readfirstlane(ent[i]);
-                const int a = (int)(e & 1023u);
-                const int bt = (int)((e >> 10) & 1023u) - 1;
-                const int sh = (int)(e >> 20);
-                const float* hrow = data + a * bins + lane;
-                // tail column c + sh before and after the wrap
-                const float* trow = data + bt * bins + lane + sh;
-                const float* twrap = trow - bins;
-                const int lim = bins - sh;
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    if (EXACT || q < kd) {
-                        const float* tp = lane + q * kWave >= lim ? twrap : trow;
-                        v[i * K + q] = hrow[q * kWave] + tp[q * kWave];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int i = 0; i < R; ++i) {
-            if (i < nr) {
-                float* orow = data + (wave + i * NW) * bins + lane;
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    if (EXACT) {
-                        if (q < K - 1 || last_ok) orow[q * kWave] = v[i * K + q];
-                    } else if (q < kd && lane + q * kWave < bins) {
-                        orow[q * kWave] = v[i * K + q];
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        rows = out_rows;
+
+    // Column q of this lane exists in a row of `bins` columns.
+    static __device__ __forceinline__ bool col_ok(int q, int kd, bool last_ok, int lane, int bins)
+    {
+        if (EXACT) return q < K - 1 || last_ok;
+        return q < kd && lane + q * kWave < bins;
     }
-    return rows;
-}
+
+    // Stages `rows` rows into LDS: LDS row p is global row grow[p] (cone
+    // tasks) or r0 + p (whole nodes).  Each wavefront issues the loads of all
+    // its rows before the first store.
+    static __device__ __forceinline__ void load(float* data, gcfloat* src, const uint32_t* grow,
+                                                int r0, int rows, int bins, int pitch, int wave,
+                                                int lane)
+    {
+        const int kd = (bins + kWave - 1) / kWave;
+        const bool last_ok = lane + (K - 1) * kWave < bins;
+        for (int base = 0; base < rows; base += R * NW) {
+            const int nr = (rows - base - wave + NW - 1) / NW;
+            float v[R * K];
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                if (i < nr) {
+                    const int p = base + wave + i * NW;
+                    const int g = grow ? (int)__builtin_amdgcn_readfirstlane(grow[p]) : r0 + p;
+                    gcfloat* srow = src + (size_t)g * bins + lane;
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
+                        if (col_ok(q, kd, last_ok, lane, bins)) v[i * K + q] = srow[q * kWave];
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                if (i < nr) {
+                    float* drow = data + (base + wave + i * NW) * pitch + lane;
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
+                        if (col_ok(q, kd, last_ok, lane, bins)) drow[q * kWave] = v[i * K + q];
+                }
+            }
+        }
+    }
+
+    // Writes LDS rows 0..rows-1 to consecutive global rows of dst.
+    static __device__ __forceinline__ void store(gfloat* dst, const float* data, int rows,
+                                                 int bins, int pitch, int wave, int lane)
+    {
+        const int kd = (bins + kWave - 1) / kWave;
+        const bool last_ok = lane + (K - 1) * kWave < bins;
+        for (int p = wave; p < rows; p += NW) {
+            const float* srow = data + p * pitch + lane;
+            gfloat* drow = dst + (size_t)p * bins + lane;
+#pragma unroll
+            for (int q = 0; q < K; ++q)
+                if (col_ok(q, kd, last_ok, lane, bins)) drow[q * kWave] = srow[q * kWave];
+        }
+    }
+
+    // All merge levels, bottom-up, in place.  Per element: two LDS reads
+    // (head column c, tail column (c + shift) mod bins, the wrap chosen by
+    // one compare against a per-row scalar) and one add; the results stay in
+    // registers until the barrier, then overwrite the level in place.  Row -1
+    // of `data` is the -0.0f row, the tail of copied rows.
+    static __device__ __forceinline__ int merge(const Seg* seg, float* data, const uint32_t* tab,
+                                                int rows, int levels, bool cone, int bins,
+                                                int pitch, int cap_rows, int wave, int lane)
+    {
+        const int kd = (bins + kWave - 1) / kWave;
+        const bool last_ok = lane + (K - 1) * kWave < bins;
+        for (int t = levels - 1; t >= 0; --t) {
+            const int out_rows = level_rows(cone, seg, t, rows);
+            const uint32_t* tl = tab + (size_t)t * cap_rows;
+            const int nr = (out_rows - wave + NW - 1) / NW;
+            // this wavefront's row-table entries, all read before the first use
+            uint32_t ent[R];
+#pragma unroll
+            for (int i = 0; i < R; ++i) ent[i] = tl[min(wave + i * NW, out_rows - 1)];
+            float v[R * K];
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                if (i < nr) {
+                    const uint32_t e = __builtin_amdgcn_readfirstlane(ent[i]);
+                    const int a = (int)(e & 1023u);
+                    const int bt = (int)((e >> 10) & 1023u) - 1;
+                    const int sh = (int)(e >> 20);
+                    const float* hrow = data + a * pitch + lane;
+                    // tail column c + sh before and after the wrap
+                    const float* trow = data + bt * pitch + lane + sh;
+                    const float* twrap = trow - bins;
+                    const int lim = bins - sh;
+#pragma unroll
+                    for (int q = 0; q < K; ++q) {
+                        if (EXACT || q < kd) {
+                            const float* tp = lane + q * kWave >= lim ? twrap : trow;
+                            v[i * K + q] = hrow[q * kWave] + tp[q * kWave];
+                        }
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                if (i < nr) {
+                    float* orow = data + (wave + i * NW) * pitch + lane;
+#pragma unroll
+                    for (int q = 0; q < K; ++q)
+                        if (col_ok(q, kd, last_ok, lane, bins)) orow[q * kWave] = v[i * K + q];
+                }
+            }
+            __syncthreads();
+            rows = out_rows;
+        }
+        return rows;
+    }
+
+    // Load, merge levels and (kind 0) store of one task whose tables are
+    // built; returns the rows of the final level.
+    static __device__ __forceinline__ int run(const FfaTask& tk, const Seg* seg, float* data,
+                                              const uint32_t* tab, const uint32_t* grow, int rows,
+                                              int pitch, int wave, int lane)
+    {
+        const int bins = tk.bins;
+        load(data, as_global(tk.src), grow, tk.node_r0, rows, bins, pitch, wave, lane);
+        __syncthreads();
+        rows = merge(seg, data, tab, rows, tk.levels, tk.cone != 0, bins, pitch, tk.pad, wave, lane);
+        if (tk.kind == 0) {
+            store(as_global(tk.dst) + ((size_t)tk.node_r0 + tk.lo) * bins, data, rows, bins, pitch,
+                  wave, lane);
+            __syncthreads();
+        }
+        return rows;
+    }
+};
 
 // Dispatch on the columns per lane: exact instantiations up to 16 columns
-// (bins <= 1024), then one per rows-per-wavefront class.
+// (bins <= 1024), then one per rows-per-wavefront class for wider rows.
 template <int NW, int STAGE>
-__device__ __forceinline__ int merge_levels(const Seg* seg, float* data, const uint32_t* tab,
-                                            int rows, int levels, bool cone, int bins,
-                                            int cap_rows, int wave, int lane)
+__device__ __forceinline__ int run_rows(const FfaTask& tk, const Seg* seg, float* data,
+                                        const uint32_t* tab, const uint32_t* grow, int rows,
+                                        int pitch, int wave, int lane)
 {
-    const int k = (bins + kWave - 1) / kWave;
-#define RT_MERGE_ARGS seg, data, tab, rows, levels, cone, bins, cap_rows, wave, lane
+    const int k = (tk.bins + kWave - 1) / kWave;
+#define RT_CORE_ARGS tk, seg, data, tab, grow, rows, pitch, wave, lane
     switch (k) {
-#define RT_MERGE_K(K)                                                                          \
+#define RT_CORE_K(K)                                                                           \
     case K:                                                                                    \
-        return merge_levels_k<NW, K, STAGE, true>(RT_MERGE_ARGS);
-    RT_MERGE_K(1) RT_MERGE_K(2) RT_MERGE_K(3) RT_MERGE_K(4) RT_MERGE_K(5) RT_MERGE_K(6)
-    RT_MERGE_K(7) RT_MERGE_K(8) RT_MERGE_K(9) RT_MERGE_K(10) RT_MERGE_K(11) RT_MERGE_K(12)
-    RT_MERGE_K(13) RT_MERGE_K(14) RT_MERGE_K(15) RT_MERGE_K(16)
-#undef RT_MERGE_K
+        return RowCore<NW, K, STAGE, true>::run(RT_CORE_ARGS);
+    RT_CORE_K(1) RT_CORE_K(2) RT_CORE_K(3) RT_CORE_K(4) RT_CORE_K(5) RT_CORE_K(6)
+    RT_CORE_K(7) RT_CORE_K(8) RT_CORE_K(9) RT_CORE_K(10) RT_CORE_K(11) RT_CORE_K(12)
+    RT_CORE_K(13) RT_CORE_K(14) RT_CORE_K(15) RT_CORE_K(16)
+#undef RT_CORE_K
     default: break;
     }
-    // wide rows: R rows per wavefront of up to STAGE/R columns
-    if (STAGE / 4 > 16 && k <= STAGE / 4)
-        return merge_levels_k<NW, (STAGE / 4 > 16 ? STAGE / 4 : 16), STAGE, false>(RT_MERGE_ARGS);
-    if (STAGE / 3 > 16 && k <= STAGE / 3)
-        return merge_levels_k<NW, (STAGE / 3 > 16 ? STAGE / 3 : 16), STAGE, false>(RT_MERGE_ARGS);
-    if (STAGE / 2 > 16 && k <= STAGE / 2)
-        return merge_levels_k<NW, (STAGE / 2 > 16 ? STAGE / 2 : 16), STAGE, false>(RT_MERGE_ARGS);
-    return merge_levels_k<NW, STAGE, STAGE, false>(RT_MERGE_ARGS);
-#undef RT_MERGE_ARGS
+    constexpr int K4 = STAGE / 4 > 16 ? STAGE / 4 : 16;
+    constexpr int K3 = STAGE / 3 > 16 ? STAGE / 3 : 16;
+    constexpr int K2 = STAGE / 2 > 16 ? STAGE / 2 : 16;
+    if (STAGE / 4 > 16 && k <= STAGE / 4) return RowCore<NW, K4, STAGE, false>::run(RT_CORE_ARGS);
+    if (STAGE / 3 > 16 && k <= STAGE / 3) return RowCore<NW, K3, STAGE, false>::run(RT_CORE_ARGS);
+    if (STAGE / 2 > 16 && k <= STAGE / 2) return RowCore<NW, K2, STAGE, false>::run(RT_CORE_ARGS);
+    return RowCore<NW, STAGE, STAGE, false>::run(RT_CORE_ARGS);
+#undef RT_CORE_ARGS
 }
 
 // Runs one FfaTask with the whole workgroup; LDS contents are dead on return.
 //
-// The row tables of all merge levels are built first (one barrier).  Every
-// level is then processed one row per wavefront: the row's packed table entry
-// is one broadcast LDS read, lanes walk the columns c = lane + 64*q, and each
-// lane keeps up to kStage results in registers until the barrier after which
-// the level is written back in place.
+// The row tables of all merge levels are built first (one barrier), then the
+// rows are staged, merged level by level and either written back to HBM or
+// turned into per-width boxcar S/N without leaving LDS.
 template <int NT, int STAGE>
-__device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
+__device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 {
     constexpr int NW = NT / kWave;
     const int bins = tk.bins;
+    const int pitch = ffa_pitch(bins);
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
     const int cap_rows = tk.pad;
-    const int k = (bins + kWave - 1) / kWave;  // columns per lane
 
     // row -1 of data is all -0.0f; rows 0..cap_rows-1 hold the staged rows
-    float* data = lds + bins;
-    uint32_t* tab = reinterpret_cast<uint32_t*>(data + (size_t)cap_rows * bins);
+    float* data = lds + pitch;
+    uint32_t* tab = reinterpret_cast<uint32_t*>(data + (size_t)cap_rows * pitch);
     Seg* seg = reinterpret_cast<Seg*>(tab + (size_t)cap_rows * kFfaMaxLevels);
     __shared__ int s_bottom;
 
@@ -353,6 +400,7 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 
     // ---- bottom rows and the tables of every level
     int rows;  // rows in the current level's layout
+    const uint32_t* grow = nullptr;
     if (tk.cone) {
         build_cone(seg, tk, &s_bottom);
         rows = (tk.levels > 0) ? s_bottom : (tk.hi - tk.lo);
@@ -364,105 +412,137 @@ __device__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
         }
         // global source row of every bottom row (in the table slot of the
         // deepest level, which no merge level uses), then every level's map
-        uint32_t* grow = tab + (size_t)(kFfaMaxLevels - 1) * cap_rows;
+        uint32_t* gr = tab + (size_t)(kFfaMaxLevels - 1) * cap_rows;
         const int L = tk.levels;
         for (int p = tid; p < rows; p += NT) {
             const Seg g = seg[seg_find(seg, L, p)];
-            grow[p] = (uint32_t)(g.r0 + g.lo + (p - g.base));
+            gr[p] = (uint32_t)(g.r0 + g.lo + (p - g.base));
         }
-        for (int t = 0; t < L; ++t) {
-            const int first = seg_slot(t, 0);
-            const int n = 1 << t;
-            const int rt = seg[first + n - 1].base + seg[first + n - 1].len;
-            for (int p = tid; p < rt; p += NT) {
+        // one flat loop over (level, row) so that every thread has work
+        for (int e = tid; e < L * cap_rows; e += NT) {
+            const int t = e / cap_rows;
+            const int p = e - t * cap_rows;
+            if (p < level_rows(true, seg, t, 0)) {
                 int a, b, sh;
                 cone_row_map(seg, t, p, bins, &a, &b, &sh);
-                tab[(size_t)t * cap_rows + p] = pack_row(a / bins, b < 0 ? -1 : b / bins, sh);
+                tab[e] = pack_row(a, b, sh);
             }
         }
-        __syncthreads();
-        load_rows<NW>(data, tk.src, grow, rows, bins, k, wave, lane);
+        grow = gr;
     } else {
         rows = tk.node_m;
-        if (tk.levels > kFfaMaxLevels) {
+        if (rows > cap_rows || tk.levels > kFfaMaxLevels) {
             if (tid == 0) atomicOr(err, 1);
             __syncthreads();
             return;
         }
-        for (int t = 0; t < tk.levels; ++t)
-            for (int p = tid; p < rows; p += NT) {
-                int a, b, sh;
-                identity_row_map(tk.node_m, t, p, bins, &a, &b, &sh);
-                tab[(size_t)t * cap_rows + p] = pack_row(a / bins, b < 0 ? -1 : b / bins, sh);
-            }
-        // contiguous rows: a flat copy with several loads in flight per lane
-        const float* src = tk.src + (size_t)tk.node_r0 * bins;
-        const int total = rows * bins;
-        for (int e0 = tid; e0 < total; e0 += NT * 8) {
-            float v[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (e0 + u * NT < total) v[u] = src[e0 + u * NT];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (e0 + u * NT < total) data[e0 + u * NT] = v[u];
+        for (int e = tid; e < tk.levels * rows; e += NT) {
+            const int t = e / rows;
+            const int p = e - t * rows;
+            int a, b, sh;
+            identity_row_map(tk.node_m, t, p, bins, &a, &b, &sh);
+            tab[(size_t)t * cap_rows + p] = pack_row(a, b, sh);
         }
     }
     __syncthreads();
 
-    // ---- merge levels, bottom-up, in place through registers
-    rows = merge_levels<NW, STAGE>(seg, data, tab, rows, tk.levels, tk.cone != 0, bins, cap_rows,
-                                   wave, lane);
-
-    // ---- outputs
-    if (tk.kind == 0) {
-        float* dst = tk.dst + ((size_t)tk.node_r0 + tk.lo) * bins;
-        const int total = rows * bins;
-        for (int e = tid; e < total; e += NT)
-            dst[e] = data[e];
-        __syncthreads();
-        return;
-    }
+    // ---- staging, merge levels (and the store of kind-0 tasks)
+    rows = run_rows<NW, STAGE>(tk, seg, data, tab, grow, rows, pitch, wave, lane);
+    if (tk.kind == 0) return;
 
     // S/N of every finished row s = lo + p with s < rows_eval, in two
-    // thread-level phases (no cross-lane traffic):
-    //  1. one thread per row turns the row into its circular prefix sums in
-    //     place, accumulating left to right in float-float (two-sum), so
-    //     every partial sum is the float32 rounding of the exact sum, as the
-    //     reference's float64 accumulator gives (kernels.hpp:73-90);
-    //  2. one thread per (row, width) scans the row for the largest window
-    //     sum and writes that width's S/N (snr.hpp:44-54).
+    // phases:
+    //  1. circular prefix sums in place (kernels.hpp:73-90): every row is cut
+    //     into C chunks, one thread per chunk; a chunk's float-float total,
+    //     an exclusive scan of the totals over the C lanes of the row, then a
+    //     rescan of the chunk from that offset writing each partial sum
+    //     rounded once to float32, as the reference's float64 accumulator
+    //     gives;
+    //  2. one wavefront per row: lane l takes the windows starting at
+    //     columns l + 64q, keeps one running maximum per width, and the
+    //     maxima are reduced across the wavefront (snr.hpp:44-54).
     const int s_end = min(tk.hi, tk.rows_eval);
     const int nrows = max(0, s_end - tk.lo);
-    for (int p = tid; p < nrows; p += NT) {
-        float* row = data + (size_t)p * bins;
+    int C = 1;
+    while (C < 32 && nrows * (2 * C) <= NT && 2 * C * 4 <= bins) C *= 2;
+    const int L = (bins + C - 1) / C;
+    for (int t0 = 0; t0 < nrows * C; t0 += NT) {
+        const int t = t0 + tid;
+        const bool active = t < nrows * C;
+        const int p = active ? t / C : 0;
+        const int c = t & (C - 1);
+        float* row = data + (size_t)p * pitch;
+        const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
         float sh = 0.0f, sl = 0.0f;
-        for (int i = 0; i < bins; ++i) {
-            float s, e;
-            two_sum(sh, row[i], s, e);
-            sh = s;
-            sl += e;
-            const float r = sh + sl;  // rounded partial sum
-            sl -= r - sh;             // keep (sh, sl) exact: sh + sl unchanged
-            sh = r;
-            row[i] = r;
+        if (active) {
+            for (int j = j0; j < j1; ++j) {
+                float s, e;
+                two_sum(sh, row[j], s, e);
+                sl += e;
+                sh = s + sl;
+                sl -= sh - s;
+            }
+        }
+        // inclusive scan of the chunk totals over the row's C lanes (C
+        // divides the wavefront, so a row never straddles two)
+        float ih = sh, il = sl;
+        for (int d = 1; d < C; d <<= 1) {
+            const float oh = __shfl_up(ih, d, C);
+            const float ol = __shfl_up(il, d, C);
+            if (c >= d) ff_add(oh, ol, ih, il, ih, il);
+        }
+        float eh = __shfl_up(ih, 1, C);
+        float el = __shfl_up(il, 1, C);
+        if (c == 0) {
+            eh = 0.0f;
+            el = 0.0f;
+        }
+        if (active) {
+            sh = eh;
+            sl = el;
+            for (int j = j0; j < j1; ++j) {
+                float s, e;
+                two_sum(sh, row[j], s, e);
+                sl += e;
+                sh = s + sl;  // rounded partial sum
+                sl -= sh - s;
+                row[j] = sh;
+            }
         }
     }
     __syncthreads();
+    // windows: one thread per (row, width), width-major so that the lanes of
+    // a wavefront scan the same columns of consecutive rows (uniform loop
+    // bounds; the odd row pitch keeps their LDS reads on distinct banks)
     const int nw = tk.nw;
+    gcfloat* hb = as_global(tk.hb);
     for (int job = tid; job < nrows * nw; job += NT) {
-        const int p = job / nw;
-        const int iw = job - p * nw;
-        const float* cps = data + (size_t)p * bins;
-        const int w = tk.widths[iw];
+        const int iw = job / nrows;
+        const int p = job - iw * nrows;
+        const float* cps = data + (size_t)p * pitch;
+        const int w = as_global(tk.widths)[iw];
         const float total = cps[bins - 1];
-        // i + w < bins: plain differences; beyond: the extension cps[j] + total
-        float dm = -INFINITY;
         const int split = bins - w;
-        for (int i = 0; i < split; ++i) dm = fmaxf(dm, cps[i + w] - cps[i]);
-        for (int i = split; i < bins; ++i) dm = fmaxf(dm, (cps[i + w - bins] + total) - cps[i]);
-        const float snr = __fmaf_rn(dm, tk.hb[2 * iw], -(tk.hb[2 * iw + 1] * total)) / tk.stdnoise;
-        tk.snr[(size_t)(tk.lo + p) * nw + iw] = snr;
+        float d[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) d[u] = -INFINITY;
+        int i = 0;
+        for (; i + 8 <= split; i += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) d[u] = fmaxf(d[u], cps[i + u + w] - cps[i + u]);
+        }
+        for (; i < split; ++i) d[0] = fmaxf(d[0], cps[i + w] - cps[i]);
+        // windows crossing the end: the extension cps[j] + total
+        for (; i + 4 <= bins; i += 4) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                d[4 + u] = fmaxf(d[4 + u], (cps[i + u - split] + total) - cps[i + u]);
+        }
+        for (; i < bins; ++i) d[1] = fmaxf(d[1], (cps[i - split] + total) - cps[i]);
+        const float dm = fmaxf(fmaxf(fmaxf(d[0], d[1]), fmaxf(d[2], d[3])),
+                               fmaxf(fmaxf(d[4], d[5]), fmaxf(d[6], d[7])));
+        const float snr = __fmaf_rn(dm, hb[2 * iw], -(hb[2 * iw + 1] * total)) / tk.stdnoise;
+        as_global(tk.snr)[(size_t)(tk.lo + p) * nw + iw] = snr;
     }
     __syncthreads();
 }

@@ -22,8 +22,10 @@ int ffa_cap_rows(int bins, int shape)
 {
     // LDS: rows of `bins` floats plus one packed table entry per row and level
     const int avail = kFfaShape[shape].lds_bytes - kSegBytes;
-    // (one more row holds the -0.0f row the copy rows add)
-    const int cap = (avail - 4 * bins) / (4 * (bins + kFfaMaxLevels));
+    // rows are ffa_pitch(bins) floats apart; one more row holds the -0.0f
+    // row the copy rows add
+    const int pitch = ffa_pitch(bins);
+    const int cap = (avail - 4 * pitch) / (4 * (pitch + kFfaMaxLevels));
     const int k = (bins + kWave - 1) / kWave;
     const int stage = kFfaShape[shape].stage;
     if (k > stage || bins >= 4096) return 0;

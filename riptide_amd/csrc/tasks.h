@@ -25,6 +25,10 @@ constexpr FfaShape kFfaShape[kFfaShapes] = {
 };
 // Largest stage over the shapes.
 constexpr int kStage = 72;
+// LDS row pitch of rows of `bins` floats: odd, so that the lanes of a
+// wavefront reading one column of consecutive rows hit distinct banks.
+constexpr int ffa_pitch(int bins) { return bins | 1; }
+
 // Rows per wavefront per merge level with k columns per lane: what the
 // staging holds, at most 16 (narrow rows would otherwise need more row-table
 // registers than values).

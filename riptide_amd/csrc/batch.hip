@@ -88,11 +88,14 @@ struct Batch {
     std::vector<int> tr_plan_index;      // transform index within its plan
     std::vector<DsSeriesDev> host_ds;
     std::vector<DsStepDev> host_steps;
+    std::vector<PrepSeriesDev> host_prep;  // per series, pointers filled at bind
+    std::vector<int64_t> work_off;         // per series, offset in the group work buffer
     // device memory
     float* d_tables = nullptr;
     FfaTransform* d_trs = nullptr;
     DsSeriesDev* d_ds_series = nullptr;
     DsStepDev* d_ds_steps = nullptr;
+    PrepSeriesDev* d_prep_series = nullptr;
     float* d_ds = nullptr;
     float* d_prep = nullptr;
     float* d_scratch = nullptr;
@@ -118,7 +121,8 @@ static void free_batch(Batch* b)
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
     for (void* p : {(void*)b->d_tables, (void*)b->d_trs, (void*)b->d_ds_series,
-                    (void*)b->d_ds_steps, (void*)b->d_ds, (void*)b->d_prep,
+                    (void*)b->d_ds_steps, (void*)b->d_prep_series, (void*)b->d_ds,
+                    (void*)b->d_prep,
                     (void*)b->d_scratch, (void*)b->d_work, (void*)b->d_queue, (void*)b->d_err})
         if (p) hipFree(p);
     delete b;
@@ -329,14 +333,31 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
         max_prep = std::max(max_prep, prep_run);
         b->groups.push_back(g);
     }
-    for (int64_t i = 0; i < nseries; ++i) {
-        if (deredden) {
-            const int64_t ws = (int64_t)std::llround(rmed_width / tsamps[i]);
-            const int sf = scrunch_factor(ws, rmed_minpts);
-            max_work = std::max<int64_t>(max_work, sf == 1 ? sizes[i] : 2 * (sizes[i] / sf));
+    // preparation work areas of every series of a group, side by side:
+    // block means, running median, normalisation partials and moments
+    b->host_prep.assign(nseries, PrepSeriesDev{});
+    b->work_off.assign(nseries, 0);
+    for (const Group& g : b->groups) {
+        int64_t run = 0;
+        for (int s = g.s0; s < g.s1; ++s) {
+            PrepSeriesDev& d = b->host_prep[s];
+            d.N = sizes[s];
+            d.sf = 1;
+            d.nl = sizes[s];
+            d.width = 1;
+            d.dereddened = deredden;
+            b->work_off[s] = run;
+            if (deredden) {
+                const int64_t ws = (int64_t)std::llround(rmed_width / tsamps[s]);
+                d.sf = scrunch_factor(ws, rmed_minpts);
+                d.nl = sizes[s] / d.sf;
+                d.width = (int32_t)(d.sf == 1 ? ws : rmed_minpts);
+                run += (d.sf == 1 ? 0 : d.nl) + d.nl;
+            }
+            run += 2 * (kNormaliseParts + 2);
         }
+        max_work = std::max(max_work, run);
     }
-    max_work += 2 * (kNormaliseParts + 2);
 
     int cus = 0;
     BT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
@@ -354,6 +375,7 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
     BT_HIP(hipMalloc(&b->d_trs, std::max<size_t>(b->host_trs.size(), 1) * sizeof(FfaTransform)));
     BT_HIP(hipMalloc(&b->d_ds_series, std::max<size_t>(b->host_ds.size(), 1) * sizeof(DsSeriesDev)));
     BT_HIP(hipMalloc(&b->d_ds_steps, std::max<size_t>(b->host_steps.size(), 1) * sizeof(DsStepDev)));
+    BT_HIP(hipMalloc(&b->d_prep_series, std::max<size_t>(b->host_prep.size(), 1) * sizeof(PrepSeriesDev)));
     BT_HIP(hipMalloc(&b->d_ds, (size_t)b->ds_floats * sizeof(float)));
     BT_HIP(hipMalloc(&b->d_prep, (size_t)b->prep_floats * sizeof(float)));
     BT_HIP(hipMalloc(&b->d_scratch, (size_t)b->slots * b->slot_floats * sizeof(float)));
@@ -393,6 +415,15 @@ int rt_batch_bind(void* handle, const float* const* h_inputs, float* const* h_sn
         b->host_ds[k].x = prep ? b->d_prep + b->prep_off[s] : b->inputs[s];
         b->host_ds[k].out = b->d_ds + b->ds_off[s];
     }
+    for (int64_t i = 0; i < b->nseries; ++i) {
+        PrepSeriesDev& d = b->host_prep[i];
+        float* w = b->d_work + b->work_off[i];
+        d.x = b->inputs[i];
+        d.out = b->d_prep + b->prep_off[i];
+        d.lores = w;
+        d.rmed = (b->deredden && d.sf > 1) ? w + d.nl : w;
+        d.stats = w + (b->deredden ? (d.sf == 1 ? 0 : d.nl) + d.nl : 0);
+    }
     for (size_t k = 0; k < b->host_trs.size(); ++k) {
         FfaTransform& f = b->host_trs[k];
         const int s = b->tr_series[k];
@@ -412,6 +443,8 @@ int rt_batch_bind(void* handle, const float* const* h_inputs, float* const* h_sn
                      hipMemcpyHostToDevice));
     BT_HIP(hipMemcpy(b->d_ds_steps, b->host_steps.data(),
                      b->host_steps.size() * sizeof(DsStepDev), hipMemcpyHostToDevice));
+    BT_HIP(hipMemcpy(b->d_prep_series, b->host_prep.data(),
+                     b->host_prep.size() * sizeof(PrepSeriesDev), hipMemcpyHostToDevice));
     b->bound = true;
     if (b->exec) {
         hipGraphExecDestroy(b->exec);
@@ -432,33 +465,19 @@ static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         const Group& g = b->groups[gi];
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 0], s));
-        // prepare: dereddening then normalisation, per series
+        // prepare: dereddening then normalisation of every series of the
+        // group in one set of launches
         if (b->deredden || b->normalise) {
+            int64_t max_n = 1, max_lores = 0;
+            int max_width = 1;
             for (int i = g.s0; i < g.s1; ++i) {
-                float* prep = b->d_prep + b->prep_off[i];
-                const float* x = b->inputs[i];
-                const int64_t N = b->N[i];
-                if (b->deredden) {
-                    const int64_t ws = (int64_t)std::llround(b->rmed_width / b->tsamp[i]);
-                    const int sf = scrunch_factor(ws, b->rmed_minpts);
-                    if (sf == 1) {
-                        BT_HIP(launch_running_median(x, N, (int)ws, b->d_work, s));
-                        BT_HIP(launch_deredden_subtract(x, N, b->d_work, N, 1, prep, s));
-                    } else {
-                        const int64_t nl = N / sf;
-                        float* lores = b->d_work;
-                        float* rmed = lores + nl;
-                        BT_HIP(launch_scrunch_mean(x, nl, sf, lores, s));
-                        BT_HIP(launch_running_median(lores, nl, (int)b->rmed_minpts, rmed, s));
-                        BT_HIP(launch_deredden_subtract(x, N, rmed, nl, sf, prep, s));
-                    }
-                    x = prep;
-                }
-                if (b->normalise) {
-                    float* work = b->d_work + (b->work_floats - 2 * (kNormaliseParts + 2));
-                    BT_HIP(launch_normalise(x, N, prep, work, kNormaliseParts, s));
-                }
+                const PrepSeriesDev& d = b->host_prep[i];
+                max_n = std::max(max_n, d.N);
+                max_lores = std::max(max_lores, d.sf > 1 ? d.nl : d.N);
+                max_width = std::max(max_width, (int)d.width);
             }
+            BT_HIP(launch_prep_batch(b->d_prep_series + g.s0, g.s1 - g.s0, max_n, max_lores,
+                                     max_width, b->deredden, b->normalise, s));
         }
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 1], s));
         if (g.total_chunks > 0)

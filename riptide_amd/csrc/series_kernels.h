@@ -35,6 +35,20 @@ struct DsSeriesDev {
     int32_t pad;
 };
 
+// One series of the batched preparation (dereddening + normalisation).
+struct PrepSeriesDev {
+    const float* x;   // input series
+    float* out;       // prepared series
+    float* lores;     // N/sf block means (sf > 1)
+    float* rmed;      // running median of lores (sf > 1) or of x (sf == 1)
+    float* stats;     // 2*kNormaliseParts partials, then mean and std pairs
+    int64_t N;
+    int64_t nl;       // N / sf
+    int64_t sf;       // scrunch factor
+    int32_t width;    // running median width
+    int32_t dereddened;  // normalise `out` (dereddened) rather than `x`
+};
+
 // Splits a positive float64 into (mf, ef) with f == mf * 2^ef, 2^52 <= mf < 2^53.
 void split_double(double f, uint64_t* mf, int* ef);
 
@@ -50,6 +64,11 @@ hipError_t launch_scrunch_mean(const float* x, int64_t nblocks, int64_t factor, 
                                hipStream_t stream);
 hipError_t launch_deredden_subtract(const float* x, int64_t N, const float* fp, int64_t nl,
                                     int64_t sf, float* out, hipStream_t stream);
+// Dereddening and/or normalisation of nseries series in a few launches
+// (series = blockIdx.y); same results as the one-series launches.
+hipError_t launch_prep_batch(const PrepSeriesDev* d_ps, int nseries, int64_t max_n,
+                             int64_t max_lores, int max_width, int deredden, int normalise,
+                             hipStream_t stream);
 // mean/std workspace: 2*(parts + 2) floats
 hipError_t launch_normalise(const float* x, int64_t N, float* out, float* work, int parts,
                             hipStream_t stream);

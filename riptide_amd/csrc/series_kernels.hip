@@ -4,6 +4,8 @@
 // the reference's binding (rollback, fused_rollback_add, circular prefix sum).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "exact_f64.h"
 #include "ffa_device.h"
 #include "series_kernels.h"
@@ -112,14 +114,14 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
 // edge-padded window, found by counting (pure comparisons, bit-exact).
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void running_median_kernel(const float* __restrict__ x,
-                                                             int64_t N, int width,
-                                                             int use_lds,
-                                                             float* __restrict__ out)
+// Median of the edge-padded window of `width` samples centred on output
+// o0 + threadIdx.x, counting ranks; `win` holds x[o0 - half .. o0 + blockDim.x
+// + half) (clamped) when use_lds, else x is read directly.
+__device__ __forceinline__ void running_median_block(const float* __restrict__ x, int64_t N,
+                                                     int width, int use_lds, int64_t o0,
+                                                     float* s_win, float* __restrict__ out)
 {
-    extern __shared__ float s_win[];
     const int half = width / 2;
-    const int64_t o0 = (int64_t)blockIdx.x * blockDim.x;
     const int span = blockDim.x + width - 1;
     if (use_lds) {
         for (int j = threadIdx.x; j < span; j += blockDim.x) {
@@ -160,6 +162,15 @@ __global__ __launch_bounds__(256) void running_median_kernel(const float* __rest
         }
     }
     out[i] = med;
+}
+
+__global__ __launch_bounds__(256) void running_median_kernel(const float* __restrict__ x,
+                                                             int64_t N, int width,
+                                                             int use_lds,
+                                                             float* __restrict__ out)
+{
+    extern __shared__ float s_win[];
+    running_median_block(x, N, width, use_lds, (int64_t)blockIdx.x * blockDim.x, s_win, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -235,14 +246,21 @@ __device__ float pairwise_sum(const float* a, int64_t n)
 }
 
 // Block means of `factor` samples: numpy float32 mean = (0 + pairwise sum)/factor.
-__global__ void scrunch_mean_kernel(const float* __restrict__ x, int64_t nblocks, int64_t factor,
-                                    float* __restrict__ out)
+__device__ __forceinline__ void scrunch_mean_range(const float* __restrict__ x, int64_t nblocks,
+                                                   int64_t factor, float* __restrict__ out,
+                                                   int64_t first, int64_t stride)
 {
-    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < nblocks;
-         b += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t b = first; b < nblocks; b += stride) {
         const float s = 0.0f + pairwise_sum(x + b * factor, factor);
         out[b] = s / (float)factor;
     }
+}
+
+__global__ void scrunch_mean_kernel(const float* __restrict__ x, int64_t nblocks, int64_t factor,
+                                    float* __restrict__ out)
+{
+    scrunch_mean_range(x, nblocks, factor, out, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                       (int64_t)gridDim.x * blockDim.x);
 }
 
 // out[i] = float32(x[i] - interp(i)) (or float32(interp(i)) when x is null),
@@ -250,12 +268,12 @@ __global__ void scrunch_mean_kernel(const float* __restrict__ x, int64_t nblocks
 // interpolation of the low-resolution running median at centres
 // j*sf + (sf-1)/2 (np.interp semantics: clamp to the end values, exact node
 // values on the nodes).  Evaluated in float-float arithmetic, rounded once.
-__global__ void deredden_subtract_kernel(const float* __restrict__ x, int64_t N,
-                                         const float* __restrict__ fp, int64_t nl, int64_t sf,
-                                         float* __restrict__ out)
+__device__ __forceinline__ void deredden_subtract_range(const float* __restrict__ x, int64_t N,
+                                                        const float* __restrict__ fp, int64_t nl,
+                                                        int64_t sf, float* __restrict__ out,
+                                                        int64_t first, int64_t stride)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = first; i < N; i += stride) {
         // 2*x_j = 2*j*sf + sf - 1; node j is the last with 2*x_j <= 2*i
         const int64_t num = 2 * i - sf + 1;
         int64_t j = (num >= 0) ? num / (2 * sf) : -1;
@@ -299,6 +317,14 @@ __global__ void deredden_subtract_kernel(const float* __restrict__ x, int64_t N,
     }
 }
 
+__global__ void deredden_subtract_kernel(const float* __restrict__ x, int64_t N,
+                                         const float* __restrict__ fp, int64_t nl, int64_t sf,
+                                         float* __restrict__ out)
+{
+    deredden_subtract_range(x, N, fp, nl, sf, out, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                            (int64_t)gridDim.x * blockDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Normalisation (time_series.py:67-93): float-float sums, fixed reduction
 // order, one rounding of (x - mean)/std.
@@ -328,18 +354,18 @@ __device__ __forceinline__ void block_ff_reduce(float& h, float& l, float* s_h, 
     }
 }
 
-// mode 0: partial sums of x; mode 1: partial sums of (x - mean)^2
-__global__ __launch_bounds__(256) void moments_partial_kernel(const float* __restrict__ x,
-                                                              int64_t N, int mode,
-                                                              const float* __restrict__ mean,
-                                                              float* __restrict__ partial)
+// mode 0: partial sums of x; mode 1: partial sums of (x - mean)^2; part
+// `part` of `nparts` (256 threads each, grid-stride over the series)
+__device__ __forceinline__ void moments_partial(const float* __restrict__ x, int64_t N, int mode,
+                                                const float* __restrict__ mean,
+                                                float* __restrict__ partial, int part, int nparts,
+                                                float* s_h, float* s_l)
 {
-    __shared__ float s_h[4], s_l[4];
     float h = 0.0f, l = 0.0f;
     const float mh = mode ? mean[0] : 0.0f;
     const float ml = mode ? mean[1] : 0.0f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = (int64_t)part * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)nparts * blockDim.x) {
         const float v = x[i];
         if (mode == 0) {
             ff_add(h, l, v, 0.0f, h, l);
@@ -354,18 +380,26 @@ __global__ __launch_bounds__(256) void moments_partial_kernel(const float* __res
     }
     block_ff_reduce(h, l, s_h, s_l);
     if (threadIdx.x == 0) {
-        partial[2 * blockIdx.x] = h;
-        partial[2 * blockIdx.x + 1] = l;
+        partial[2 * part] = h;
+        partial[2 * part + 1] = l;
     }
+}
+
+__global__ __launch_bounds__(256) void moments_partial_kernel(const float* __restrict__ x,
+                                                              int64_t N, int mode,
+                                                              const float* __restrict__ mean,
+                                                              float* __restrict__ partial)
+{
+    __shared__ float s_h[4], s_l[4];
+    moments_partial(x, N, mode, mean, partial, blockIdx.x, gridDim.x, s_h, s_l);
 }
 
 // Reduces the partials; mode 0 writes mean = sum/N, mode 1 writes
 // std = sqrt(sum/N), both as float-float pairs.
-__global__ __launch_bounds__(256) void moments_final_kernel(const float* __restrict__ partial,
-                                                            int nparts, int64_t N, int mode,
-                                                            float* __restrict__ result)
+__device__ __forceinline__ void moments_final(const float* __restrict__ partial, int nparts,
+                                              int64_t N, int mode, float* __restrict__ result,
+                                              float* s_h, float* s_l)
 {
-    __shared__ float s_h[4], s_l[4];
     float h = 0.0f, l = 0.0f;
     for (int i = threadIdx.x; i < nparts; i += blockDim.x)
         ff_add(h, l, partial[2 * i], partial[2 * i + 1], h, l);
@@ -398,15 +432,24 @@ __global__ __launch_bounds__(256) void moments_final_kernel(const float* __restr
     }
 }
 
+__global__ __launch_bounds__(256) void moments_final_kernel(const float* __restrict__ partial,
+                                                            int nparts, int64_t N, int mode,
+                                                            float* __restrict__ result)
+{
+    __shared__ float s_h[4], s_l[4];
+    moments_final(partial, nparts, N, mode, result, s_h, s_l);
+}
+
 // out = float32((x - mean) / std) with float-float mean and std
-__global__ void normalise_apply_kernel(const float* __restrict__ x, int64_t N,
-                                       const float* __restrict__ mean,
-                                       const float* __restrict__ stdv, float* __restrict__ out)
+__device__ __forceinline__ void normalise_apply_range(const float* __restrict__ x, int64_t N,
+                                                      const float* __restrict__ mean,
+                                                      const float* __restrict__ stdv,
+                                                      float* __restrict__ out, int64_t first,
+                                                      int64_t stride)
 {
     const float mh = mean[0], ml = mean[1];
     const float sh = stdv[0], sl = stdv[1];
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = first; i < N; i += stride) {
         float dh, dl;
         two_sum(x[i], -mh, dh, dl);
         dl -= ml;
@@ -417,6 +460,83 @@ __global__ void normalise_apply_kernel(const float* __restrict__ x, int64_t N,
         const float r = ((dh - p) - pe) + dl - q * sl;
         out[i] = q + r / sh;
     }
+}
+
+__global__ void normalise_apply_kernel(const float* __restrict__ x, int64_t N,
+                                       const float* __restrict__ mean,
+                                       const float* __restrict__ stdv, float* __restrict__ out)
+{
+    normalise_apply_range(x, N, mean, stdv, out, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                          (int64_t)gridDim.x * blockDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// Batched preparation: the kernels above over many series at once, series
+// = blockIdx.y, same per-series arithmetic and partition as the one-series
+// launches (bit-identical results).
+// ---------------------------------------------------------------------------
+
+__global__ void prep_scrunch_kernel(const PrepSeriesDev* __restrict__ ps)
+{
+    const PrepSeriesDev p = ps[blockIdx.y];
+    if (p.sf <= 1) return;
+    scrunch_mean_range(p.x, p.nl, p.sf, p.lores, (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                       (int64_t)gridDim.x * blockDim.x);
+}
+
+__global__ __launch_bounds__(256) void prep_rmed_kernel(const PrepSeriesDev* __restrict__ ps,
+                                                        int use_lds)
+{
+    extern __shared__ float s_win[];
+    const PrepSeriesDev p = ps[blockIdx.y];
+    const float* x = p.sf > 1 ? p.lores : p.x;
+    const int64_t n = p.sf > 1 ? p.nl : p.N;
+    for (int64_t o0 = (int64_t)blockIdx.x * blockDim.x; o0 < n;
+         o0 += (int64_t)gridDim.x * blockDim.x) {
+        running_median_block(x, n, p.width, use_lds, o0, s_win, p.rmed);
+        __syncthreads();
+    }
+}
+
+__global__ void prep_subtract_kernel(const PrepSeriesDev* __restrict__ ps)
+{
+    const PrepSeriesDev p = ps[blockIdx.y];
+    deredden_subtract_range(p.x, p.N, p.rmed, p.sf > 1 ? p.nl : p.N, p.sf, p.out,
+                            (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                            (int64_t)gridDim.x * blockDim.x);
+}
+
+// normalisation input: the dereddened series (out) or the raw one
+__device__ __forceinline__ const float* prep_norm_src(const PrepSeriesDev& p)
+{
+    return p.dereddened ? p.out : p.x;
+}
+
+__global__ __launch_bounds__(256) void prep_moments_partial_kernel(
+    const PrepSeriesDev* __restrict__ ps, int mode)
+{
+    __shared__ float s_h[4], s_l[4];
+    const PrepSeriesDev p = ps[blockIdx.y];
+    moments_partial(prep_norm_src(p), p.N, mode, p.stats + 2 * kNormaliseParts, p.stats,
+                    blockIdx.x, gridDim.x, s_h, s_l);
+}
+
+__global__ __launch_bounds__(256) void prep_moments_final_kernel(
+    const PrepSeriesDev* __restrict__ ps, int mode)
+{
+    __shared__ float s_h[4], s_l[4];
+    const PrepSeriesDev p = ps[blockIdx.y];
+    moments_final(p.stats, kNormaliseParts, p.N, mode, p.stats + 2 * kNormaliseParts + 2 * mode,
+                  s_h, s_l);
+}
+
+__global__ void prep_normalise_apply_kernel(const PrepSeriesDev* __restrict__ ps, int grid_x)
+{
+    const PrepSeriesDev p = ps[blockIdx.y];
+    normalise_apply_range(prep_norm_src(p), p.N, p.stats + 2 * kNormaliseParts,
+                          p.stats + 2 * kNormaliseParts + 2, p.out,
+                          (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                          (int64_t)grid_x * blockDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -565,6 +685,41 @@ hipError_t launch_normalise(const float* x, int64_t N, float* out, float* work, 
                        stdv);
     hipLaunchKernelGGL(normalise_apply_kernel, dim3(grid_for(N, 256)), dim3(256), 0, stream, x, N,
                        (const float*)mean, (const float*)stdv, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_prep_batch(const PrepSeriesDev* d_ps, int nseries, int64_t max_n,
+                            int64_t max_lores, int max_width, int deredden, int normalise,
+                            hipStream_t stream)
+{
+    if (nseries <= 0) return hipSuccess;
+    if (deredden) {
+        if (max_lores > 0) {
+            hipLaunchKernelGGL(prep_scrunch_kernel, dim3(grid_for(max_lores, 64, 4096), nseries),
+                               dim3(64), 0, stream, d_ps);
+        }
+        // the window of a workgroup in LDS when it fits in 48 KiB
+        const size_t span = (size_t)256 + max_width - 1;
+        const int use_lds = span * sizeof(float) <= 48 * 1024;
+        hipLaunchKernelGGL(prep_rmed_kernel,
+                           dim3(grid_for(std::max(max_lores, (int64_t)1), 256, 4096), nseries),
+                           dim3(256), use_lds ? span * sizeof(float) : 0, stream, d_ps, use_lds);
+        hipLaunchKernelGGL(prep_subtract_kernel, dim3(grid_for(max_n, 256, 2048), nseries), dim3(256),
+                           0, stream, d_ps);
+    }
+    if (normalise) {
+        hipLaunchKernelGGL(prep_moments_partial_kernel, dim3(kNormaliseParts, nseries), dim3(256),
+                           0, stream, d_ps, 0);
+        hipLaunchKernelGGL(prep_moments_final_kernel, dim3(1, nseries), dim3(256), 0, stream, d_ps,
+                           0);
+        hipLaunchKernelGGL(prep_moments_partial_kernel, dim3(kNormaliseParts, nseries), dim3(256),
+                           0, stream, d_ps, 1);
+        hipLaunchKernelGGL(prep_moments_final_kernel, dim3(1, nseries), dim3(256), 0, stream, d_ps,
+                           1);
+        const int gx = grid_for(max_n, 256);
+        hipLaunchKernelGGL(prep_normalise_apply_kernel, dim3(gx, nseries), dim3(256), 0, stream,
+                           d_ps, gx);
+    }
     return hipGetLastError();
 }
 

@@ -149,6 +149,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain copy kernel over 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -159,6 +161,20 @@ def main():
 
     import torch
     from riptide_amd import _lib, libcpp
+
+    if args.calibrate_copy:
+        # rollback_kernel with shift 0: reads and writes 2^28 floats (1 GiB
+        # each way) once per launch, five launches
+        n = 1 << 28
+        src = torch.randn(n, device=_lib.device())
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            _lib.call("rt_rollback", _lib.ptr(src), n, 0, _lib.ptr(dst), _lib.stream())
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_kernel": "rollback_kernel", "bytes_read": 4 * n,
+                          "bytes_written": 4 * n, "launches": 5}), flush=True)
+        return
+
     from riptide_amd.batch import BatchSearch
     from riptide_amd.ffautils import generate_width_trials
 
@@ -262,6 +278,17 @@ def main():
     lds_bytes = 8 * st["ffa_adds"]
     lds_achieved = lds_bytes / (phases["ffa_ms"] * 1e-3) / 1e9
 
+    # HBM bytes per launch of the same kernel and workload, measured with two
+    # rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a copy-kernel
+    # calibration; committed under profiles/ (the counters cannot be read
+    # from inside this process)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "ffa_batch_kernel__batch512x2p22_traffic.json")
+    if (os.path.exists(tpath) and args.trials == 512 and args.nsamp == 2 ** 22
+            and args.group_size == 64):
+        with open(tpath) as f:
+            traffic = int(json.load(f)["hbm_bytes_per_launch"])
+
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -303,7 +330,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_source": ("profiles/ffa_batch_kernel__batch512x2p22_traffic.json"
+                               if traffic is not None else None),
             "algorithmic_bytes_per_launch": int(bytes_per_launch),
             "ms_per_launch": round(ms_per_launch, 3),
             "lds": {"achieved": round(lds_achieved, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",

@@ -30,6 +30,9 @@ HIP_FLAGS = [
     # is written; the fused multiply-adds the reference build makes are
     # written out with __fmaf_rn where they matter for parity
     "-ffp-contract=off",
+    # no signalling-NaN quieting in min/max (the reference is built with
+    # -ffinite-math-only): fmaxf compiles to one v_max_f32
+    "-mno-amdgpu-ieee", "-fno-honor-nans",
     "-Wall", "-Wno-unused-function",
 ]
 GCC_FLAGS = [

@@ -56,112 +56,137 @@ __device__ __forceinline__ int seg_find(const Seg* seg, int t, int p)
     return first + lo;
 }
 
-// Builds the cone of a tile top-down: level 0 is the tile itself; level t+1
-// holds the rows of the children that level t reads.  Returns R_levels (rows
-// at the bottom) in *bottom_rows.  Must be called by the whole workgroup.
-__device__ __forceinline__ void build_cone(Seg* seg, const FfaTask& tk, int* shared_int)
+// Merge coefficients k = (rows_child - 1)/(M - 1) (float32 division, as
+// transforms.hpp:17-18) of every node size of a merge tree: the nodes at
+// relative depth t of an M-row node have (M >> t) or (M >> t) + 1 rows, so
+// two entries per depth; entry 2t + (m - (M >> t)).
+struct MergeCoef {
+    float kh[2 * (kFfaMaxLevels + 1)];
+    float kt[2 * (kFfaMaxLevels + 1)];
+};
+
+__device__ __forceinline__ void fill_merge_coef(MergeCoef* mc, int M, int levels, int lane)
 {
-    const int tid = threadIdx.x;
-    if (tid == 0) {
-        Seg s;
-        s.m = tk.node_m;
-        s.r0 = tk.node_r0;
-        s.lo = tk.lo;
-        s.len = tk.hi - tk.lo;
-        s.base = 0;
-        seg[0] = s;
+    if (lane < 2 * (levels + 1)) {
+        const int t = lane >> 1;
+        const int m = (M >> t) + (lane & 1);
+        const int mh = m >> 1;
+        const int mt = m - mh;
+        mc->kh[lane] = m > 1 ? ((float)mh - 1.0f) / ((float)m - 1.0f) : 0.0f;
+        mc->kt[lane] = m > 1 ? ((float)mt - 1.0f) / ((float)m - 1.0f) : 0.0f;
     }
-    __syncthreads();
+}
+
+__device__ __forceinline__ int coef_slot(int M, int t, int m) { return 2 * t + (m - (M >> t)); }
+
+// Builds the cone of a tile top-down with one wavefront (no workgroup
+// barrier): level 0 is the tile itself; level t+1 holds the rows of the
+// children that level t reads.  Writes R_levels (rows at the bottom) to
+// *bottom_rows.  A level has at most 2^kMaxConeLevels slots, two per lane.
+__device__ __forceinline__ void build_cone_wave(Seg* seg, const FfaTask& tk, const MergeCoef* mc,
+                                                int* bottom_rows, int lane)
+{
+    static_assert((2 << (kMaxConeLevels - 1)) <= 2 * kWave, "cone level wider than a wavefront");
+    const int M = tk.node_m;
+    if (lane == 0) {
+        Seg r;
+        r.m = M;
+        r.r0 = tk.node_r0;
+        r.lo = tk.lo;
+        r.len = tk.hi - tk.lo;
+        r.base = 0;
+        seg[0] = r;
+    }
+    int bottom = tk.hi - tk.lo;
     for (int t = 0; t < tk.levels; ++t) {
-        const int nslots = 1 << t;
-        for (int j = tid; j < nslots; j += blockDim.x) {
+        // the slots of level t were written by other lanes of this wavefront
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const int j = lane;  // parent slot (2^t <= 64)
+        Seg a, b;
+        a.m = 0; a.r0 = 0; a.lo = 0; a.len = 0; a.base = 0;
+        b = a;
+        if (j < (1 << t)) {
             const Seg p = seg[seg_slot(t, j)];
-            Seg a, b;
-            a.m = 0; a.r0 = p.r0; a.lo = 0; a.len = 0; a.base = 0;
-            b = a;
+            a.r0 = p.r0;
+            b.r0 = p.r0;
             if (p.len > 0 && p.m == 1) {
                 a = p;  // single-row leaf: carried down unchanged
             } else if (p.len > 0 && p.m >= 2) {
                 const int mh = p.m >> 1;
                 const int mt = p.m - mh;
-                const float kh = ((float)mh - 1.0f) / ((float)p.m - 1.0f);
-                const float kt = ((float)mt - 1.0f) / ((float)p.m - 1.0f);
+                const int ks = coef_slot(M, t, p.m);
+                const float kh = mc->kh[ks];
+                const float kt = mc->kt[ks];
                 const int s0 = p.lo, s1 = p.lo + p.len - 1;
                 const int h0 = merge_row(kh, s0), h1 = merge_row(kh, s1);
                 const int t0 = merge_row(kt, s0), t1 = merge_row(kt, s1);
                 a.m = mh; a.r0 = p.r0; a.lo = h0; a.len = h1 - h0 + 1;
                 b.m = mt; b.r0 = p.r0 + mh; b.lo = t0; b.len = t1 - t0 + 1;
             }
+        }
+        // exclusive scan of the children's lengths, children 2j and 2j+1 on lane j
+        const int run = a.len + b.len;
+        int incl = run;
+        for (int d = 1; d < kWave; d <<= 1) {
+            const int o = __shfl_up(incl, d, kWave);
+            if (lane >= d) incl += o;
+        }
+        a.base = incl - run;
+        b.base = a.base + a.len;
+        if (j < (1 << t)) {
             seg[seg_slot(t + 1, 2 * j)] = a;
             seg[seg_slot(t + 1, 2 * j + 1)] = b;
         }
-        __syncthreads();
-        // exclusive scan of the children's lengths (<= 2^kMaxConeLevels slots)
-        if (tid < kWave) {
-            const int n = 2 << t;
-            const int per = (n + kWave - 1) / kWave;
-            const int first = seg_slot(t + 1, 0);
-            int run = 0;
-            for (int q = 0; q < per; ++q) {
-                const int j = tid * per + q;
-                if (j < n) run += seg[first + j].len;
-            }
-            int incl = run;
-            for (int d = 1; d < kWave; d <<= 1) {
-                const int o = __shfl_up(incl, d, kWave);
-                if (tid >= d) incl += o;
-            }
-            int excl = incl - run;
-            for (int q = 0; q < per; ++q) {
-                const int j = tid * per + q;
-                if (j < n) {
-                    seg[first + j].base = excl;
-                    excl += seg[first + j].len;
-                }
-            }
-            if (tid == kWave - 1) *shared_int = incl;
-        }
-        __syncthreads();
+        bottom = __shfl(incl, kWave - 1, kWave);
     }
+    if (lane == 0) *bottom_rows = bottom;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
 
-// Row map of one merge level of a whole-node (identity layout) task: row p of
-// the node at relative depth t reads head row *a and tail row *b (-1: none)
-// rolled left by *sh.
-__device__ __forceinline__ void identity_row_map(int node_m, int t, int p, int bins, int* a,
-                                                 int* b, int* sh)
+// Packed per-row table entry of one merge level: head row (10 bits), tail
+// row + 1 (10 bits, 0 = copy the head row) and circular shift (12 bits).
+__device__ __forceinline__ uint32_t pack_row(int a, int b, int sh)
 {
-    int r0 = 0, m = node_m;
-    for (int d = 0; d < t && m > 1; ++d) {
-        const int mh = m >> 1;
-        if (p < r0 + mh) {
-            m = mh;
+    return (uint32_t)a | ((uint32_t)(b + 1) << 10) | ((uint32_t)sh << 20);
+}
+
+// Row-table entries of every merge level for row p of a whole-node task (the
+// node at relative depth t holding row p reads head row a and tail row b,
+// rolled left by sh), walking the tree down from the root once.
+__device__ __forceinline__ void identity_row_tables(int M, int levels, int p, int bins,
+                                                    const MergeCoef* mc, uint32_t* tab,
+                                                    int cap_rows)
+{
+    int r0 = 0, m = M;
+    for (int t = 0; t < levels; ++t) {
+        uint32_t e;
+        if (m <= 1) {
+            e = pack_row(p, -1, 0);
         } else {
-            r0 += mh;
-            m -= mh;
+            const int mh = m >> 1;
+            const int ks = coef_slot(M, t, m);
+            const int s = p - r0;
+            const int h = merge_row(mc->kh[ks], s);
+            const int tt = merge_row(mc->kt[ks], s);
+            int sh = s - tt;
+            while (sh >= bins) sh -= bins;
+            e = pack_row(r0 + h, r0 + mh + tt, sh);
+            if (s < mh) {
+                m = mh;
+            } else {
+                r0 += mh;
+                m -= mh;
+            }
         }
+        tab[(size_t)t * cap_rows + p] = e;
     }
-    if (m <= 1) {
-        *a = p;
-        *b = -1;
-        *sh = 0;
-        return;
-    }
-    const int mh = m >> 1;
-    const int mt = m - mh;
-    const float kh = ((float)mh - 1.0f) / ((float)m - 1.0f);
-    const float kt = ((float)mt - 1.0f) / ((float)m - 1.0f);
-    const int s = p - r0;
-    const int h = merge_row(kh, s);
-    const int tt = merge_row(kt, s);
-    *a = r0 + h;
-    *b = r0 + mh + tt;
-    *sh = (s - tt) % bins;
 }
 
 // Row map of one merge level of a cone task (level t reads level t+1).
-__device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int bins, int* a,
-                                             int* b, int* sh)
+__device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int bins, int M,
+                                             const MergeCoef* mc, int* a, int* b, int* sh)
 {
     const int slot = seg_find(seg, t, p);
     const Seg g = seg[slot];
@@ -175,23 +200,16 @@ __device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int b
         *sh = 0;
         return;
     }
-    const int mh = g.m >> 1;
-    const int mt = g.m - mh;
-    const float kh = ((float)mh - 1.0f) / ((float)g.m - 1.0f);
-    const float kt = ((float)mt - 1.0f) / ((float)g.m - 1.0f);
-    const int h = merge_row(kh, s);
-    const int tt = merge_row(kt, s);
+    const int ks = coef_slot(M, t, g.m);
+    const int h = merge_row(mc->kh[ks], s);
+    const int tt = merge_row(mc->kt[ks], s);
     *a = ca.base + (h - ca.lo);
     *b = cb.base + (tt - cb.lo);
-    *sh = (s - tt) % bins;
+    int r = s - tt;
+    while (r >= bins) r -= bins;
+    *sh = r;
 }
 
-// Packed per-row table entry of one merge level: head row (10 bits), tail
-// row + 1 (10 bits, 0 = copy the head row) and circular shift (12 bits).
-__device__ __forceinline__ uint32_t pack_row(int a, int b, int sh)
-{
-    return (uint32_t)a | ((uint32_t)(b + 1) << 10) | ((uint32_t)sh << 20);
-}
 
 // Rows in the layout of merge level t of a task (the rows that level writes).
 __device__ __forceinline__ int level_rows(bool cone, const Seg* seg, int t, int rows)
@@ -202,6 +220,14 @@ __device__ __forceinline__ int level_rows(bool cone, const Seg* seg, int t, int 
     return seg[first + n - 1].base + seg[first + n - 1].len;
 }
 
+// 32-bit LDS address of a pointer into LDS, and a load from one.
+typedef __attribute__((address_space(3))) const float lds_cfloat;
+__device__ __forceinline__ uint32_t lds_address(const float* p)
+{
+    return (uint32_t)(uintptr_t)(lds_cfloat*)p;
+}
+__device__ __forceinline__ float lds_load(uint32_t a) { return *(lds_cfloat*)(uintptr_t)a; }
+
 // Everything of a task that depends on the row length, with at most K
 // columns per lane (bins <= 64*K; exactly K when EXACT) known at compile time.
 // LDS rows are `pitch` floats apart (odd, so that lanes reading the same
@@ -211,6 +237,8 @@ template <int NW, int K, int STAGE, bool EXACT>
 struct RowCore {
     static constexpr int R = ffa_rows_per_wave(K, STAGE);
     static_assert(R >= 1, "column count beyond the register staging");
+    // rows per branch-free group of the merge loop (about 12 loads in flight)
+    static constexpr int G = K >= 12 ? 1 : K >= 6 ? 2 : K >= 3 ? 4 : 8;
 
     // Column q of this lane exists in a row of `bins` columns.
     static __device__ __forceinline__ bool col_ok(int q, int kd, bool last_ok, int lane, int bins)
@@ -289,23 +317,38 @@ struct RowCore {
 #pragma unroll
             for (int i = 0; i < R; ++i) ent[i] = tl[min(wave + i * NW, out_rows - 1)];
             float v[R * K];
+            // rows in groups of G without a branch between them, so that the
+            // loads of a whole group are in flight together (a group may
+            // run past this wavefront's last row: its entries are clamped to
+            // a real row and its results are never written)
 #pragma unroll
-            for (int i = 0; i < R; ++i) {
-                if (i < nr) {
-                    const uint32_t e = __builtin_amdgcn_readfirstlane(ent[i]);
-                    const int a = (int)(e & 1023u);
-                    const int bt = (int)((e >> 10) & 1023u) - 1;
-                    const int sh = (int)(e >> 20);
-                    const float* hrow = data + a * pitch + lane;
-                    // tail column c + sh before and after the wrap
-                    const float* trow = data + bt * pitch + lane + sh;
-                    const float* twrap = trow - bins;
-                    const int lim = bins - sh;
+            for (int i0 = 0; i0 < R; i0 += G) {
+                if (i0 < nr) {
 #pragma unroll
-                    for (int q = 0; q < K; ++q) {
-                        if (EXACT || q < kd) {
-                            const float* tp = lane + q * kWave >= lim ? twrap : trow;
-                            v[i * K + q] = hrow[q * kWave] + tp[q * kWave];
+                    for (int g = 0; g < G; ++g) {
+                        const int i = i0 + g;
+                        if (i >= R) break;
+                        const uint32_t e = __builtin_amdgcn_readfirstlane(ent[i]);
+                        const int a = (int)(e & 1023u);
+                        const int bt = (int)((e >> 10) & 1023u) - 1;
+                        const int sh = (int)(e >> 20);
+                        const float* hrow = data + a * pitch + lane;
+                        // LDS addresses of the tail column c + sh before and
+                        // after the wrap; the second is hidden from the
+                        // optimiser so that each element costs one compare
+                        // and one select, the column step staying an
+                        // immediate offset
+                        const uint32_t t1 = lds_address(data + bt * pitch + lane + sh);
+                        uint32_t t2 = t1 - bins * 4;
+                        asm volatile("" : "+v"(t2));
+                        const int lim = bins - sh;
+#pragma unroll
+                        for (int q = 0; q < K; ++q) {
+                            if (EXACT || q < kd) {
+                                const uint32_t tb = lane + q * kWave >= lim ? t2 : t1;
+                                const float tv = lds_load(tb + q * kWave * 4);
+                                v[i * K + q] = hrow[q * kWave] + tv;
+                            }
                         }
                     }
                 }
@@ -332,10 +375,12 @@ struct RowCore {
                                               const uint32_t* tab, const uint32_t* grow, int rows,
                                               int pitch, int wave, int lane)
     {
-        const int bins = tk.bins;
-        load(data, as_global(tk.src), grow, tk.node_r0, rows, bins, pitch, wave, lane);
+        const int bins = __builtin_amdgcn_readfirstlane(tk.bins);
+        load(data, as_global(tk.src), grow, __builtin_amdgcn_readfirstlane(tk.node_r0), rows, bins,
+             pitch, wave, lane);
         __syncthreads();
-        rows = merge(seg, data, tab, rows, tk.levels, tk.cone != 0, bins, pitch, tk.pad, wave, lane);
+        rows = merge(seg, data, tab, rows, __builtin_amdgcn_readfirstlane(tk.levels), tk.cone != 0,
+                     bins, pitch, __builtin_amdgcn_readfirstlane(tk.pad), wave, lane);
         if (tk.kind == 0) {
             store(as_global(tk.dst) + ((size_t)tk.node_r0 + tk.lo) * bins, data, rows, bins, pitch,
                   wave, lane);
@@ -374,6 +419,227 @@ __device__ __forceinline__ int run_rows(const FfaTask& tk, const Seg* seg, float
 #undef RT_CORE_ARGS
 }
 
+// Order-preserving map of a float to an int (and back: the map is its own
+// inverse), so that LDS integer max gives the float max.
+__device__ __forceinline__ int ordered_int(float f)
+{
+    const int i = __float_as_int(f);
+    return i ^ ((i >> 31) & 0x7FFFFFFF);
+}
+__device__ __forceinline__ float ordered_float(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
+
+// Window maxima of one chunk [i0, i1) of one row for NWC widths (snr.hpp:44-54
+// per width: dmax = max_i cps[i+w] - cps[i] over the circular extension
+// cps[bins + j] = cps[j] + total).  Every lane of a wavefront works on the
+// same chunk of a different row, so all branches are uniform.
+template <int NWC>
+__device__ __forceinline__ void window_max_chunk(const float* cps, int bins, int i0, int i1,
+                                                 const int (&w)[kSnrMaxWidths], float total,
+                                                 float (&m)[kSnrMaxWidths])
+{
+    const int wmax = w[NWC - 1];
+    // columns whose windows all stay inside the row: each prefix value is
+    // read once for all widths
+    const int plain_end = max(i0, min(i1, bins - wmax));
+    int i = i0;
+    for (; i + 4 <= plain_end; i += 4) {
+        float c[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) c[u] = cps[i + u];
+#pragma unroll
+        for (int j = 0; j < NWC; ++j) {
+            const float* q = cps + i + w[j];
+            m[j] = fmaxf(m[j], fmaxf(fmaxf(q[0] - c[0], q[1] - c[1]),
+                                     fmaxf(q[2] - c[2], q[3] - c[3])));
+        }
+    }
+    for (; i < plain_end; ++i) {
+        const float c0 = cps[i];
+#pragma unroll
+        for (int j = 0; j < NWC; ++j) m[j] = fmaxf(m[j], cps[i + w[j]] - c0);
+    }
+    if (i >= i1) return;
+    // the last wmax columns: width by width, windows inside the row then the
+    // ones running past its end (i, the widths and the bounds are the same on
+    // every lane, so the loops are uniform)
+    const int t0 = i;
+#pragma unroll
+    for (int j = 0; j < NWC; ++j) {
+        const int wj = w[j];
+        const int split = max(t0, min(i1, bins - wj));
+        float d0 = m[j], d1 = -INFINITY;
+        int k = t0;
+        for (; k + 2 <= split; k += 2)
+            d0 = fmaxf(d0, fmaxf(cps[k + wj] - cps[k], cps[k + 1 + wj] - cps[k + 1]));
+        for (; k < split; ++k) d0 = fmaxf(d0, cps[k + wj] - cps[k]);
+        for (; k + 2 <= i1; k += 2)
+            d1 = fmaxf(d1, fmaxf((cps[k + wj - bins] + total) - cps[k],
+                                 (cps[k + 1 + wj - bins] + total) - cps[k + 1]));
+        for (; k < i1; ++k) d1 = fmaxf(d1, (cps[k + wj - bins] + total) - cps[k]);
+        m[j] = fmaxf(d0, d1);
+    }
+}
+
+// First column of window chunk c of C, balancing the cost per row: a column
+// whose windows stay inside the row costs (1 + nw) reads, one of the last
+// wmax columns 2*nw.
+__device__ __forceinline__ int window_chunk_start(int c, int C, int bins, int wmax, int nw)
+{
+    if (c <= 0) return 0;
+    if (c >= C) return bins;
+    const int B = max(0, bins - wmax);
+    const int a = 1 + nw, t = 2 * nw;
+    const int total = B * a + (bins - B) * t;
+    const int y = (int)(((int64_t)total * c) / C);
+    return y <= B * a ? y / a : min(bins, B + (y - B * a) / t);
+}
+
+// Per-width boxcar S/N of `nrows` finished rows staged in LDS (row p at
+// data + p*pitch), written to tk.snr.  `scratch` is free LDS.
+//  1. circular prefix sums in place (kernels.hpp:73-90): each row is cut
+//     into C chunks; one thread per (chunk, row), chunk-major so that the
+//     lanes of a wavefront take the same chunk of consecutive rows.  Chunk
+//     totals in float-float go to LDS, then every chunk is rescanned from
+//     the float-float sum of the chunks before it, writing each partial sum
+//     rounded once to float32 -- what the reference's float64 accumulator
+//     gives.
+//  2. window maxima: the same (chunk, row) threads, all widths per thread
+//     (each prefix value read once per width), combined across chunks with
+//     LDS integer max; then one thread per (row, width) writes the S/N.
+template <int NT>
+__device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pitch, int nrows,
+                                         float* scratch, int scratch_floats)
+{
+    constexpr int NW = NT / kWave;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int bins = tk.bins;
+    const int nw = tk.nw;
+    if (nrows <= 0) return;
+    // (chunk, row) work items: wavefront item wc covers rows 64*(wc % wpc) ..
+    // of chunk wc / wpc; lanes past the last row repeat it and store nothing
+    const int wpc = (nrows + kWave - 1) / kWave;  // wavefronts per chunk
+    int C = max(1, NW / wpc);
+    C = min(C, max(1, bins / 8));                      // chunks of >= 8 columns
+    C = min(C, max(1, scratch_floats / (2 * nrows)));  // chunk totals fit
+    const int L = (bins + C - 1) / C;
+    float* tot = scratch;  // [C][nrows] float-float pairs
+    for (int wc = wave; wc < C * wpc; wc += NW) {
+        const int c = wc / wpc;
+        const int pr = (wc - c * wpc) * kWave + lane;
+        const int p = min(pr, nrows - 1);
+        const float* row = data + (size_t)p * pitch;
+        const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
+        float sh = 0.0f, sl = 0.0f;
+        for (int j = j0; j < j1; ++j) {
+            float s, e;
+            two_sum(sh, row[j], s, e);
+            sl += e;
+            sh = s + sl;
+            sl -= sh - s;
+        }
+        if (pr < nrows) {
+            tot[2 * (c * nrows + p)] = sh;
+            tot[2 * (c * nrows + p) + 1] = sl;
+        }
+    }
+    __syncthreads();
+    for (int wc = wave; wc < C * wpc; wc += NW) {
+        const int c = wc / wpc;
+        const int pr = (wc - c * wpc) * kWave + lane;
+        const int p = min(pr, nrows - 1);
+        float* row = data + (size_t)p * pitch;
+        float sh = 0.0f, sl = 0.0f;
+        for (int cc = 0; cc < c; ++cc)
+            ff_add(sh, sl, tot[2 * (cc * nrows + p)], tot[2 * (cc * nrows + p) + 1], sh, sl);
+        const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
+        const bool own = pr < nrows;
+        for (int j = j0; j < j1; ++j) {
+            float s, e;
+            two_sum(sh, row[j], s, e);
+            sl += e;
+            sh = s + sl;  // rounded partial sum
+            sl -= sh - s;
+            if (own) row[j] = sh;
+        }
+    }
+    __syncthreads();
+
+    gcfloat* hb = as_global(tk.hb);
+    gcint* gw = as_global(tk.widths);
+    int* mx = reinterpret_cast<int*>(scratch);  // [nrows][nw] window maxima
+    if (nw >= 1 && nw <= kSnrMaxWidths && nrows * nw <= scratch_floats) {
+        for (int e = tid; e < nrows * nw; e += NT) mx[e] = ordered_int(-INFINITY);
+        int w[kSnrMaxWidths];
+#pragma unroll
+        for (int j = 0; j < kSnrMaxWidths; ++j)
+            w[j] = __builtin_amdgcn_readfirstlane(gw[j < nw ? j : nw - 1]);
+        const int wmax = __builtin_amdgcn_readfirstlane(gw[nw - 1]);
+        __syncthreads();
+        for (int wc = wave; wc < C * wpc; wc += NW) {
+            const int c = wc / wpc;
+            const int pr = (wc - c * wpc) * kWave + lane;
+            const int p = min(pr, nrows - 1);
+            const float* cps = data + (size_t)p * pitch;
+            const float total = cps[bins - 1];
+            const int i0 = window_chunk_start(c, C, bins, wmax, nw);
+            const int i1 = window_chunk_start(c + 1, C, bins, wmax, nw);
+            float m[kSnrMaxWidths];
+#pragma unroll
+            for (int j = 0; j < kSnrMaxWidths; ++j) m[j] = -INFINITY;
+            switch (nw) {
+#define RT_WIN(N) \
+    case N: window_max_chunk<N>(cps, bins, i0, i1, w, total, m); break;
+                RT_WIN(1) RT_WIN(2) RT_WIN(3) RT_WIN(4) RT_WIN(5) RT_WIN(6) RT_WIN(7) RT_WIN(8)
+                RT_WIN(9) RT_WIN(10) RT_WIN(11) RT_WIN(12) RT_WIN(13) RT_WIN(14) RT_WIN(15)
+                RT_WIN(16)
+#undef RT_WIN
+                default: break;
+            }
+            if (pr < nrows && i0 < i1) {
+#pragma unroll
+                for (int j = 0; j < kSnrMaxWidths; ++j)
+                    if (j < nw) atomicMax(&mx[p * nw + j], ordered_int(m[j]));
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < nrows * nw; e += NT) {
+            const int p = e / nw;
+            const int iw = e - p * nw;
+            const float total = data[(size_t)p * pitch + bins - 1];
+            const float dm = ordered_float(mx[e]);
+            const float snr = __fmaf_rn(dm, hb[2 * iw], -(hb[2 * iw + 1] * total)) / tk.stdnoise;
+            as_global(tk.snr)[(size_t)(tk.lo + p) * nw + iw] = snr;
+        }
+        __syncthreads();
+        return;
+    }
+    // many widths: one thread per (row, width), width-major (uniform loop
+    // bounds; the odd row pitch keeps the lanes' LDS reads on distinct banks)
+    for (int job = tid; job < nrows * nw; job += NT) {
+        const int iw = job / nrows;
+        const int p = job - iw * nrows;
+        const float* cps = data + (size_t)p * pitch;
+        const int w = gw[iw];
+        const float total = cps[bins - 1];
+        const int split = bins - w;
+        float d0 = -INFINITY, d1 = -INFINITY;
+        int i = 0;
+        for (; i + 2 <= split; i += 2) {
+            d0 = fmaxf(d0, cps[i + w] - cps[i]);
+            d1 = fmaxf(d1, cps[i + 1 + w] - cps[i + 1]);
+        }
+        for (; i < split; ++i) d0 = fmaxf(d0, cps[i + w] - cps[i]);
+        // windows crossing the end: the extension cps[j] + total
+        for (; i < bins; ++i) d1 = fmaxf(d1, (cps[i - split] + total) - cps[i]);
+        const float dm = fmaxf(d0, d1);
+        const float snr = __fmaf_rn(dm, hb[2 * iw], -(hb[2 * iw + 1] * total)) / tk.stdnoise;
+        as_global(tk.snr)[(size_t)(tk.lo + p) * nw + iw] = snr;
+    }
+    __syncthreads();
+}
+
 // Runs one FfaTask with the whole workgroup; LDS contents are dead on return.
 //
 // The row tables of all merge levels are built first (one barrier), then the
@@ -383,12 +649,13 @@ template <int NT, int STAGE>
 __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
 {
     constexpr int NW = NT / kWave;
-    const int bins = tk.bins;
+    // task fields are uniform: keep them in scalar registers
+    const int bins = __builtin_amdgcn_readfirstlane(tk.bins);
     const int pitch = ffa_pitch(bins);
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
-    const int cap_rows = tk.pad;
+    const int cap_rows = __builtin_amdgcn_readfirstlane(tk.pad);
 
     // row -1 of data is all -0.0f; rows 0..cap_rows-1 hold the staged rows
     float* data = lds + pitch;
@@ -396,16 +663,34 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     Seg* seg = reinterpret_cast<Seg*>(tab + (size_t)cap_rows * kFfaMaxLevels);
     __shared__ int s_bottom;
 
+    __shared__ MergeCoef s_coef;
+    const int levels = __builtin_amdgcn_readfirstlane(tk.levels);
+    const int M = __builtin_amdgcn_readfirstlane(tk.node_m);
+    if (levels > kFfaMaxLevels || (tk.cone && levels > kMaxConeLevels) ||
+        (!tk.cone && M > cap_rows)) {
+        // the host sizes tasks so this cannot happen; never overrun LDS
+        if (tid == 0) atomicOr(err, 1);
+        return;
+    }
+
     for (int c = tid; c < bins; c += NT) lds[c] = -0.0f;
 
-    // ---- bottom rows and the tables of every level
+    // ---- merge coefficients, the cone (one wavefront), then the row tables
+    // of every level
+    if (wave == 0) {
+        fill_merge_coef(&s_coef, M, levels, lane);
+        if (tk.cone) {
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            build_cone_wave(seg, tk, &s_coef, &s_bottom, lane);
+        }
+    }
+    __syncthreads();
     int rows;  // rows in the current level's layout
     const uint32_t* grow = nullptr;
     if (tk.cone) {
-        build_cone(seg, tk, &s_bottom);
-        rows = (tk.levels > 0) ? s_bottom : (tk.hi - tk.lo);
-        if (rows > cap_rows || tk.levels > kFfaMaxLevels) {
-            // the host sizes tiles so this cannot happen; never overrun LDS
+        rows = (levels > 0) ? s_bottom : (tk.hi - tk.lo);
+        if (rows > cap_rows) {
             if (tid == 0) atomicOr(err, 1);
             __syncthreads();
             return;
@@ -413,36 +698,25 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
         // global source row of every bottom row (in the table slot of the
         // deepest level, which no merge level uses), then every level's map
         uint32_t* gr = tab + (size_t)(kFfaMaxLevels - 1) * cap_rows;
-        const int L = tk.levels;
         for (int p = tid; p < rows; p += NT) {
-            const Seg g = seg[seg_find(seg, L, p)];
+            const Seg g = seg[seg_find(seg, levels, p)];
             gr[p] = (uint32_t)(g.r0 + g.lo + (p - g.base));
         }
         // one flat loop over (level, row) so that every thread has work
-        for (int e = tid; e < L * cap_rows; e += NT) {
+        for (int e = tid; e < levels * cap_rows; e += NT) {
             const int t = e / cap_rows;
             const int p = e - t * cap_rows;
             if (p < level_rows(true, seg, t, 0)) {
                 int a, b, sh;
-                cone_row_map(seg, t, p, bins, &a, &b, &sh);
+                cone_row_map(seg, t, p, bins, M, &s_coef, &a, &b, &sh);
                 tab[e] = pack_row(a, b, sh);
             }
         }
         grow = gr;
     } else {
-        rows = tk.node_m;
-        if (rows > cap_rows || tk.levels > kFfaMaxLevels) {
-            if (tid == 0) atomicOr(err, 1);
-            __syncthreads();
-            return;
-        }
-        for (int e = tid; e < tk.levels * rows; e += NT) {
-            const int t = e / rows;
-            const int p = e - t * rows;
-            int a, b, sh;
-            identity_row_map(tk.node_m, t, p, bins, &a, &b, &sh);
-            tab[(size_t)t * cap_rows + p] = pack_row(a, b, sh);
-        }
+        rows = M;
+        for (int p = tid; p < rows; p += NT)
+            identity_row_tables(M, levels, p, bins, &s_coef, tab, cap_rows);
     }
     __syncthreads();
 
@@ -450,101 +724,14 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     rows = run_rows<NW, STAGE>(tk, seg, data, tab, grow, rows, pitch, wave, lane);
     if (tk.kind == 0) return;
 
-    // S/N of every finished row s = lo + p with s < rows_eval, in two
-    // phases:
-    //  1. circular prefix sums in place (kernels.hpp:73-90): every row is cut
-    //     into C chunks, one thread per chunk; a chunk's float-float total,
-    //     an exclusive scan of the totals over the C lanes of the row, then a
-    //     rescan of the chunk from that offset writing each partial sum
-    //     rounded once to float32, as the reference's float64 accumulator
-    //     gives;
-    //  2. one wavefront per row: lane l takes the windows starting at
-    //     columns l + 64q, keeps one running maximum per width, and the
-    //     maxima are reduced across the wavefront (snr.hpp:44-54).
     const int s_end = min(tk.hi, tk.rows_eval);
     const int nrows = max(0, s_end - tk.lo);
-    int C = 1;
-    while (C < 32 && nrows * (2 * C) <= NT && 2 * C * 4 <= bins) C *= 2;
-    const int L = (bins + C - 1) / C;
-    for (int t0 = 0; t0 < nrows * C; t0 += NT) {
-        const int t = t0 + tid;
-        const bool active = t < nrows * C;
-        const int p = active ? t / C : 0;
-        const int c = t & (C - 1);
-        float* row = data + (size_t)p * pitch;
-        const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
-        float sh = 0.0f, sl = 0.0f;
-        if (active) {
-            for (int j = j0; j < j1; ++j) {
-                float s, e;
-                two_sum(sh, row[j], s, e);
-                sl += e;
-                sh = s + sl;
-                sl -= sh - s;
-            }
-        }
-        // inclusive scan of the chunk totals over the row's C lanes (C
-        // divides the wavefront, so a row never straddles two)
-        float ih = sh, il = sl;
-        for (int d = 1; d < C; d <<= 1) {
-            const float oh = __shfl_up(ih, d, C);
-            const float ol = __shfl_up(il, d, C);
-            if (c >= d) ff_add(oh, ol, ih, il, ih, il);
-        }
-        float eh = __shfl_up(ih, 1, C);
-        float el = __shfl_up(il, 1, C);
-        if (c == 0) {
-            eh = 0.0f;
-            el = 0.0f;
-        }
-        if (active) {
-            sh = eh;
-            sl = el;
-            for (int j = j0; j < j1; ++j) {
-                float s, e;
-                two_sum(sh, row[j], s, e);
-                sl += e;
-                sh = s + sl;  // rounded partial sum
-                sl -= sh - s;
-                row[j] = sh;
-            }
-        }
-    }
-    __syncthreads();
-    // windows: one thread per (row, width), width-major so that the lanes of
-    // a wavefront scan the same columns of consecutive rows (uniform loop
-    // bounds; the odd row pitch keeps their LDS reads on distinct banks)
-    const int nw = tk.nw;
-    gcfloat* hb = as_global(tk.hb);
-    for (int job = tid; job < nrows * nw; job += NT) {
-        const int iw = job / nrows;
-        const int p = job - iw * nrows;
-        const float* cps = data + (size_t)p * pitch;
-        const int w = as_global(tk.widths)[iw];
-        const float total = cps[bins - 1];
-        const int split = bins - w;
-        float d[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) d[u] = -INFINITY;
-        int i = 0;
-        for (; i + 8 <= split; i += 8) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) d[u] = fmaxf(d[u], cps[i + u + w] - cps[i + u]);
-        }
-        for (; i < split; ++i) d[0] = fmaxf(d[0], cps[i + w] - cps[i]);
-        // windows crossing the end: the extension cps[j] + total
-        for (; i + 4 <= bins; i += 4) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                d[4 + u] = fmaxf(d[4 + u], (cps[i + u - split] + total) - cps[i + u]);
-        }
-        for (; i < bins; ++i) d[1] = fmaxf(d[1], (cps[i - split] + total) - cps[i]);
-        const float dm = fmaxf(fmaxf(fmaxf(d[0], d[1]), fmaxf(d[2], d[3])),
-                               fmaxf(fmaxf(d[4], d[5]), fmaxf(d[6], d[7])));
-        const float snr = __fmaf_rn(dm, hb[2 * iw], -(hb[2 * iw + 1] * total)) / tk.stdnoise;
-        as_global(tk.snr)[(size_t)(tk.lo + p) * nw + iw] = snr;
-    }
-    __syncthreads();
+    // LDS past the S/N rows is free: the other staged rows, the row tables
+    // and the cone segments
+    float* scratch = data + (size_t)nrows * pitch;
+    const int scratch_floats =
+        (cap_rows - nrows) * pitch + kFfaMaxLevels * cap_rows + (2 << kMaxConeLevels) * 5;
+    snr_rows<NT>(tk, data, pitch, nrows, scratch, scratch_floats);
 }
 
 // One workgroup per task (the task list of one pass).

@@ -25,6 +25,10 @@ constexpr FfaShape kFfaShape[kFfaShapes] = {
 };
 // Largest stage over the shapes.
 constexpr int kStage = 72;
+// Widths whose window maxima a thread of the S/N phase keeps in registers
+// (more widths take a slower path).
+constexpr int kSnrMaxWidths = 16;
+
 // LDS row pitch of rows of `bins` floats: odd, so that the lanes of a
 // wavefront reading one column of consecutive rows hit distinct banks.
 constexpr int ffa_pitch(int bins) { return bins | 1; }

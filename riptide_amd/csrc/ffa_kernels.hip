@@ -428,29 +428,39 @@ __device__ __forceinline__ int ordered_int(float f)
 }
 __device__ __forceinline__ float ordered_float(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
 
-// Window maxima of one chunk [i0, i1) of one row for NWC widths (snr.hpp:44-54
-// per width: dmax = max_i cps[i+w] - cps[i] over the circular extension
-// cps[bins + j] = cps[j] + total).  Every lane of a wavefront works on the
-// same chunk of a different row, so all branches are uniform.
+// Window maxima over window starts [i0, i1) of one row for NWC widths, all
+// windows inside the row (i1 + wmax <= bins): each prefix value is read once
+// for all widths.  Every lane of a wavefront works on the same columns of a
+// different row, so all branches are uniform.
 template <int NWC>
-__device__ __forceinline__ void window_max_chunk(const float* cps, int bins, int i0, int i1,
-                                                 const int (&w)[kSnrMaxWidths], float total,
+__device__ __forceinline__ void window_max_chunk(const float* cps, int i0, int i1,
+                                                 const int (&w)[kSnrMaxWidths],
                                                  float (&m)[kSnrMaxWidths])
 {
-    const int wmax = w[NWC - 1];
-    // columns whose windows all stay inside the row: each prefix value is
-    // read once for all widths
-    const int plain_end = max(i0, min(i1, bins - wmax));
+    const int plain_end = i1;
     int i = i0;
     for (; i + 4 <= plain_end; i += 4) {
         float c[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) c[u] = cps[i + u];
+        // widths four at a time: all loads of a group issued before the
+        // first use (the empty asm keeps the compiler from interleaving
+        // each load with its use, which would wait on every load)
 #pragma unroll
-        for (int j = 0; j < NWC; ++j) {
-            const float* q = cps + i + w[j];
-            m[j] = fmaxf(m[j], fmaxf(fmaxf(q[0] - c[0], q[1] - c[1]),
-                                     fmaxf(q[2] - c[2], q[3] - c[3])));
+        for (int j0 = 0; j0 < NWC; j0 += 4) {
+            float q[4][4];
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+                if (j0 + jj < NWC) {
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) q[jj][u] = cps[i + u + w[j0 + jj]];
+                }
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int jj = 0; jj < 4; ++jj)
+                if (j0 + jj < NWC)
+                    m[j0 + jj] = fmaxf(m[j0 + jj], fmaxf(fmaxf(q[jj][0] - c[0], q[jj][1] - c[1]),
+                                                         fmaxf(q[jj][2] - c[2], q[jj][3] - c[3])));
         }
     }
     for (; i < plain_end; ++i) {
@@ -458,40 +468,6 @@ __device__ __forceinline__ void window_max_chunk(const float* cps, int bins, int
 #pragma unroll
         for (int j = 0; j < NWC; ++j) m[j] = fmaxf(m[j], cps[i + w[j]] - c0);
     }
-    if (i >= i1) return;
-    // the last wmax columns: width by width, windows inside the row then the
-    // ones running past its end (i, the widths and the bounds are the same on
-    // every lane, so the loops are uniform)
-    const int t0 = i;
-#pragma unroll
-    for (int j = 0; j < NWC; ++j) {
-        const int wj = w[j];
-        const int split = max(t0, min(i1, bins - wj));
-        float d0 = m[j], d1 = -INFINITY;
-        int k = t0;
-        for (; k + 2 <= split; k += 2)
-            d0 = fmaxf(d0, fmaxf(cps[k + wj] - cps[k], cps[k + 1 + wj] - cps[k + 1]));
-        for (; k < split; ++k) d0 = fmaxf(d0, cps[k + wj] - cps[k]);
-        for (; k + 2 <= i1; k += 2)
-            d1 = fmaxf(d1, fmaxf((cps[k + wj - bins] + total) - cps[k],
-                                 (cps[k + 1 + wj - bins] + total) - cps[k + 1]));
-        for (; k < i1; ++k) d1 = fmaxf(d1, (cps[k + wj - bins] + total) - cps[k]);
-        m[j] = fmaxf(d0, d1);
-    }
-}
-
-// First column of window chunk c of C, balancing the cost per row: a column
-// whose windows stay inside the row costs (1 + nw) reads, one of the last
-// wmax columns 2*nw.
-__device__ __forceinline__ int window_chunk_start(int c, int C, int bins, int wmax, int nw)
-{
-    if (c <= 0) return 0;
-    if (c >= C) return bins;
-    const int B = max(0, bins - wmax);
-    const int a = 1 + nw, t = 2 * nw;
-    const int total = B * a + (bins - B) * t;
-    const int y = (int)(((int64_t)total * c) / C);
-    return y <= B * a ? y / a : min(bins, B + (y - B * a) / t);
 }
 
 // Per-width boxcar S/N of `nrows` finished rows staged in LDS (row p at
@@ -508,7 +484,8 @@ __device__ __forceinline__ int window_chunk_start(int c, int C, int bins, int wm
 //     LDS integer max; then one thread per (row, width) writes the S/N.
 template <int NT>
 __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pitch, int nrows,
-                                         float* scratch, int scratch_floats)
+                                         float* scratch, int scratch_floats, int* s_w,
+                                         float* s_hb)
 {
     constexpr int NW = NT / kWave;
     const int tid = threadIdx.x;
@@ -525,6 +502,15 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
     C = min(C, max(1, scratch_floats / (2 * nrows)));  // chunk totals fit
     const int L = (bins + C - 1) / C;
     float* tot = scratch;  // [C][nrows] float-float pairs
+    // widths and (h + b, b) pairs of the transform: loaded now, stored to
+    // LDS after the first prefix pass, so that their latency is hidden
+    const int nwl = min(nw, kSnrMaxWidths);
+    int w_ld = 0;
+    float hb_ld = 0.0f;
+    if (wave == NW - 1) {
+        if (lane < nwl) w_ld = as_global(tk.widths)[lane];
+        if (lane < 2 * nwl) hb_ld = as_global(tk.hb)[lane];
+    }
     for (int wc = wave; wc < C * wpc; wc += NW) {
         const int c = wc / wpc;
         const int pr = (wc - c * wpc) * kWave + lane;
@@ -532,7 +518,22 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         const float* row = data + (size_t)p * pitch;
         const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
         float sh = 0.0f, sl = 0.0f;
-        for (int j = j0; j < j1; ++j) {
+        int j = j0;
+        for (; j + 4 <= j1; j += 4) {
+            float x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = row[j + u];
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float s, e;
+                two_sum(sh, x[u], s, e);
+                sl += e;
+                sh = s + sl;
+                sl -= sh - s;
+            }
+        }
+        for (; j < j1; ++j) {
             float s, e;
             two_sum(sh, row[j], s, e);
             sl += e;
@@ -543,6 +544,10 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             tot[2 * (c * nrows + p)] = sh;
             tot[2 * (c * nrows + p) + 1] = sl;
         }
+    }
+    if (wave == NW - 1) {
+        if (lane < nwl) s_w[lane] = w_ld;
+        if (lane < 2 * nwl) s_hb[lane] = hb_ld;
     }
     __syncthreads();
     for (int wc = wave; wc < C * wpc; wc += NW) {
@@ -555,7 +560,27 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             ff_add(sh, sl, tot[2 * (cc * nrows + p)], tot[2 * (cc * nrows + p) + 1], sh, sl);
         const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
         const bool own = pr < nrows;
-        for (int j = j0; j < j1; ++j) {
+        int j = j0;
+        for (; j + 4 <= j1; j += 4) {
+            float x[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = row[j + u];
+            asm volatile("" ::: "memory");
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                float s, e;
+                two_sum(sh, x[u], s, e);
+                sl += e;
+                sh = s + sl;  // rounded partial sum
+                sl -= sh - s;
+                x[u] = sh;
+            }
+            if (own) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) row[j + u] = x[u];
+            }
+        }
+        for (; j < j1; ++j) {
             float s, e;
             two_sum(sh, row[j], s, e);
             sl += e;
@@ -571,37 +596,79 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
     int* mx = reinterpret_cast<int*>(scratch);  // [nrows][nw] window maxima
     if (nw >= 1 && nw <= kSnrMaxWidths && nrows * nw <= scratch_floats) {
         for (int e = tid; e < nrows * nw; e += NT) mx[e] = ordered_int(-INFINITY);
+        // widths and (h + b, b) pairs were copied to LDS at the start of
+        // the task
         int w[kSnrMaxWidths];
 #pragma unroll
         for (int j = 0; j < kSnrMaxWidths; ++j)
-            w[j] = __builtin_amdgcn_readfirstlane(gw[j < nw ? j : nw - 1]);
-        const int wmax = __builtin_amdgcn_readfirstlane(gw[nw - 1]);
+            w[j] = __builtin_amdgcn_readfirstlane(s_w[j < nw ? j : nw - 1]);
+        const int wmax = __builtin_amdgcn_readfirstlane(s_w[nw - 1]);
         __syncthreads();
+        // (a) window starts [0, bins - wmax), whose windows all stay inside
+        // the row, in C chunks of equal cost, all widths per thread
+        const int P = max(0, bins - wmax);
+        const int Lw = (P + C - 1) / C;
         for (int wc = wave; wc < C * wpc; wc += NW) {
             const int c = wc / wpc;
             const int pr = (wc - c * wpc) * kWave + lane;
             const int p = min(pr, nrows - 1);
             const float* cps = data + (size_t)p * pitch;
-            const float total = cps[bins - 1];
-            const int i0 = window_chunk_start(c, C, bins, wmax, nw);
-            const int i1 = window_chunk_start(c + 1, C, bins, wmax, nw);
+            const int i0 = min(P, c * Lw), i1 = min(P, i0 + Lw);
+            if (i0 >= i1) continue;
             float m[kSnrMaxWidths];
 #pragma unroll
             for (int j = 0; j < kSnrMaxWidths; ++j) m[j] = -INFINITY;
             switch (nw) {
 #define RT_WIN(N) \
-    case N: window_max_chunk<N>(cps, bins, i0, i1, w, total, m); break;
+    case N: window_max_chunk<N>(cps, i0, i1, w, m); break;
                 RT_WIN(1) RT_WIN(2) RT_WIN(3) RT_WIN(4) RT_WIN(5) RT_WIN(6) RT_WIN(7) RT_WIN(8)
                 RT_WIN(9) RT_WIN(10) RT_WIN(11) RT_WIN(12) RT_WIN(13) RT_WIN(14) RT_WIN(15)
                 RT_WIN(16)
 #undef RT_WIN
                 default: break;
             }
-            if (pr < nrows && i0 < i1) {
+            if (pr < nrows) {
 #pragma unroll
                 for (int j = 0; j < kSnrMaxWidths; ++j)
                     if (j < nw) atomicMax(&mx[p * nw + j], ordered_int(m[j]));
             }
+        }
+        // (b) the last wmax window starts, one thread per (row, width),
+        // width-major: windows inside the row, then the ones running past its
+        // end over the extension cps[j] + total
+        for (int job = tid; job < nrows * nw; job += NT) {
+            const int iw = job / nrows;
+            const int p = job - iw * nrows;
+            const float* cps = data + (size_t)p * pitch;
+            const int wj = s_w[iw];
+            const float total = cps[bins - 1];
+            const int split = bins - wj;
+            float d0 = -INFINITY, d1 = -INFINITY;
+            int k = P;
+            for (; k + 4 <= split; k += 4) {
+                float a[4], b[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    a[u] = cps[k + u];
+                    b[u] = cps[k + u + wj];
+                }
+                asm volatile("" ::: "memory");
+                d0 = fmaxf(d0, fmaxf(fmaxf(b[0] - a[0], b[1] - a[1]), fmaxf(b[2] - a[2], b[3] - a[3])));
+            }
+            for (; k < split; ++k) d0 = fmaxf(d0, cps[k + wj] - cps[k]);
+            for (; k + 4 <= bins; k += 4) {
+                float a[4], b[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    a[u] = cps[k + u];
+                    b[u] = cps[k + u - split];
+                }
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for (int u = 0; u < 4; ++u) d1 = fmaxf(d1, (b[u] + total) - a[u]);
+            }
+            for (; k < bins; ++k) d1 = fmaxf(d1, (cps[k - split] + total) - cps[k]);
+            atomicMax(&mx[p * nw + iw], ordered_int(fmaxf(d0, d1)));
         }
         __syncthreads();
         for (int e = tid; e < nrows * nw; e += NT) {
@@ -609,7 +676,8 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             const int iw = e - p * nw;
             const float total = data[(size_t)p * pitch + bins - 1];
             const float dm = ordered_float(mx[e]);
-            const float snr = __fmaf_rn(dm, hb[2 * iw], -(hb[2 * iw + 1] * total)) / tk.stdnoise;
+            const float snr =
+                __fmaf_rn(dm, s_hb[2 * iw], -(s_hb[2 * iw + 1] * total)) / tk.stdnoise;
             as_global(tk.snr)[(size_t)(tk.lo + p) * nw + iw] = snr;
         }
         __syncthreads();
@@ -675,6 +743,9 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
 
     for (int c = tid; c < bins; c += NT) lds[c] = -0.0f;
 
+    __shared__ int s_w[kSnrMaxWidths];
+    __shared__ float s_hb[2 * kSnrMaxWidths];
+
     // ---- merge coefficients, the cone (one wavefront), then the row tables
     // of every level
     if (wave == 0) {
@@ -731,7 +802,7 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     float* scratch = data + (size_t)nrows * pitch;
     const int scratch_floats =
         (cap_rows - nrows) * pitch + kFfaMaxLevels * cap_rows + (2 << kMaxConeLevels) * 5;
-    snr_rows<NT>(tk, data, pitch, nrows, scratch, scratch_floats);
+    snr_rows<NT>(tk, data, pitch, nrows, scratch, scratch_floats, s_w, s_hb);
 }
 
 // One workgroup per task (the task list of one pass).

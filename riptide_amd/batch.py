@@ -10,7 +10,7 @@ import ctypes
 
 import numpy as np
 
-from . import _lib
+from . import _lib, tune
 from .ffautils import generate_width_trials
 from .periodogram import Periodogram
 
@@ -21,7 +21,7 @@ class BatchSearch:
 
     def __init__(self, sizes, tsamps, bins_min, bins_max, period_min=1.0, period_max=30.0,
                  widths=None, ducy_max=0.20, wtsp=1.5, deredden=True, rmed_width=4.0,
-                 rmed_minpts=101, already_normalised=False, group_size=64):
+                 rmed_minpts=101, already_normalised=False, group_size=64, shape=None):
         n = len(sizes)
         self.n = n
         self.sizes = np.ascontiguousarray(sizes, dtype=np.int64)
@@ -37,11 +37,22 @@ class BatchSearch:
         self.period_min = float(period_min)
         self.period_max = float(period_max)
         h = ctypes.c_void_p()
-        _lib.call("rt_batch_create", n, _lib.host_ptr(self.sizes), _lib.host_ptr(self.tsamps),
-                  _lib.host_ptr(flat), _lib.host_ptr(counts), self.period_min, self.period_max,
-                  _lib.host_ptr(self.bins_min), _lib.host_ptr(self.bins_max), int(deredden),
-                  float(rmed_width), int(rmed_minpts), int(not already_normalised),
-                  int(group_size), ctypes.byref(h))
+        # FFA workgroup shape: explicit, else the autotuner's choice for this
+        # bins range (riptide_amd.tune), else the library default
+        if shape is None:
+            shape = tune.shape_for("batch", int(self.bins_min.min()))
+        self.shape = shape
+        if shape is not None:
+            _lib.call("rt_set_ffa_shape", int(shape))
+        try:
+            _lib.call("rt_batch_create", n, _lib.host_ptr(self.sizes), _lib.host_ptr(self.tsamps),
+                      _lib.host_ptr(flat), _lib.host_ptr(counts), self.period_min,
+                      self.period_max, _lib.host_ptr(self.bins_min), _lib.host_ptr(self.bins_max),
+                      int(deredden), float(rmed_width), int(rmed_minpts),
+                      int(not already_normalised), int(group_size), ctypes.byref(h))
+        finally:
+            if shape is not None:
+                tune.restore()
         self.handle = h
         self.lengths = []
         for i in range(n):

@@ -8,7 +8,7 @@ HBM, runs the HIP kernels and copies the result back.
 """
 import numpy as np
 
-from . import _lib
+from . import _lib, tune
 
 
 def _f32(arr, ndim):
@@ -173,9 +173,17 @@ def periodogram_device(d_data, size, tsamp, widths, period_min, period_max, bins
     foldbins = _lib.empty((L,), "uint32")
     snrs = _lib.empty((L, w.size))
     name = "rt_periodogram_plain" if plain else "rt_periodogram"
-    _lib.call(name, _lib.ptr(d_data), int(size), float(tsamp), _lib.host_ptr(w), w.size,
-              float(period_min), float(period_max), int(bins_min), int(bins_max),
-              _lib.ptr(periods), _lib.ptr(foldbins), _lib.ptr(snrs), _lib.stream())
+    # FFA workgroup shape chosen by the autotuner for this bins range, if any
+    shape = tune.shape_for("periodogram", int(bins_min))
+    if shape is not None:
+        _lib.call("rt_set_ffa_shape", shape)
+    try:
+        _lib.call(name, _lib.ptr(d_data), int(size), float(tsamp), _lib.host_ptr(w), w.size,
+                  float(period_min), float(period_max), int(bins_min), int(bins_max),
+                  _lib.ptr(periods), _lib.ptr(foldbins), _lib.ptr(snrs), _lib.stream())
+    finally:
+        if shape is not None:
+            tune.restore()
     return periods, foldbins, snrs
 
 

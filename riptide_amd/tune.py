@@ -1,0 +1,169 @@
+"""Autotuner of the FFA kernels' workgroup shape.
+
+The FFA task kernels come in a few workgroup shapes (threads per workgroup,
+LDS per workgroup, merge-level values each lane keeps in registers; see
+``kFfaShape`` in csrc/tasks.h).  The shape fixes how many rows a workgroup
+tiles (so how many passes a transform needs) and how many workgroups share a
+compute unit.  Which one is fastest depends on the bins range, the series
+length and the schedule (one series or a batch), so it is measured:
+
+    python -m riptide_amd.tune [--nsamp N] [--tsamp T] [--trials K] [--out FILE]
+
+times, for every shape and bins class, (a) the single-series periodogram
+and (b) a batch of K series, on synthetic white noise of the given shape,
+and writes the winners to a JSON file (default: ``tuning.json`` next to this
+module, or ``$RIPTIDE_AMD_TUNING``).  ``shape_for(kind, bins_min)`` reads it;
+``BatchSearch`` and the periodogram entry points apply it when no shape was
+set explicitly with ``set_shape``.
+"""
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+
+from . import _lib
+
+NUM_SHAPES = 3
+DEFAULT_CLASSES = ((240, 260), (480, 520))
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+_cache = None
+_forced = None
+
+
+def tuning_path():
+    return os.environ.get("RIPTIDE_AMD_TUNING", os.path.join(_HERE, "tuning.json"))
+
+
+def set_shape(shape):
+    """Forces workgroup shape `shape` (0 .. NUM_SHAPES-1) for every later
+    call; None returns to the tuned (or default) choice."""
+    global _forced
+    if shape is not None and not 0 <= int(shape) < NUM_SHAPES:
+        raise ValueError(f"unknown FFA workgroup shape {shape}")
+    _forced = None if shape is None else int(shape)
+    _lib.call("rt_set_ffa_shape", -1 if shape is None else int(shape))
+
+
+def restore():
+    """Returns the library to the forced shape, or to its default."""
+    _lib.call("rt_set_ffa_shape", -1 if _forced is None else _forced)
+
+
+def load(path=None):
+    """The tuning table (empty when none was written)."""
+    global _cache
+    path = path or tuning_path()
+    if _cache is None or _cache[0] != path:
+        table = {}
+        if os.path.exists(path):
+            with open(path) as f:
+                table = json.load(f)
+        _cache = (path, table)
+    return _cache[1]
+
+
+def shape_for(kind, bins_min):
+    """Tuned shape for schedule `kind` ("batch" or "periodogram") and the bins
+    class holding bins_min (the class with the nearest lower bound), or None."""
+    if _forced is not None:
+        return _forced
+    best = load().get(kind, {})
+    if not best:
+        return None
+    key = min(best, key=lambda k: abs(int(k.split("-")[0]) - int(bins_min)))
+    return int(best[key])
+
+
+def apply(kind, bins_min):
+    """Selects the tuned shape for the next launches (no-op without a table)."""
+    s = shape_for(kind, bins_min)
+    if s is not None and _forced is None:
+        _lib.call("rt_set_ffa_shape", s)
+    return s
+
+
+def _time_periodogram(shape, nsamp, tsamp, bins, pmin, pmax, reps=3):
+    import torch
+    from . import libcpp
+    from .ffautils import generate_width_trials
+    set_shape(shape)
+    x = torch.randn(nsamp, device=_lib.device())
+    w = generate_width_trials(bins[0])
+    libcpp.periodogram_device(x, nsamp, tsamp, w, pmin, pmax, bins[0], bins[1])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        libcpp.periodogram_device(x, nsamp, tsamp, w, pmin, pmax, bins[0], bins[1])
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
+def _time_batch(shape, nsamp, tsamp, bins, pmin, pmax, trials):
+    import torch
+    from .batch import BatchSearch
+    set_shape(shape)
+    xs = [torch.randn(nsamp, device=_lib.device()) for _ in range(trials)]
+    b = BatchSearch([nsamp] * trials, tsamp, bins[0], bins[1], pmin, pmax, deredden=False,
+                    already_normalised=True, group_size=trials, shape=shape)
+    outs = b.allocate_outputs()
+    b.bind(xs, outs)
+    b.run_timed()
+    ms = min(b.run_timed()["ffa_ms"] for _ in range(2))
+    b.close()
+    return ms
+
+
+def tune(nsamp=2 ** 22, tsamp=64e-6, period_min=0.1, period_max=10.0, trials=16,
+         classes=DEFAULT_CLASSES, out=None, log=print):
+    """Measures every shape on every bins class and writes the table."""
+    table = {"device": None, "workload": {"nsamp": nsamp, "tsamp": tsamp,
+                                          "period_min": period_min, "period_max": period_max,
+                                          "trials": trials},
+             "periodogram": {}, "batch": {}, "timings_ms": {}}
+    try:
+        import torch
+        table["device"] = torch.cuda.get_device_name(0)
+    except Exception:
+        pass
+    for bins in classes:
+        key = f"{bins[0]}-{bins[1]}"
+        try:
+            pg = [_time_periodogram(s, nsamp, tsamp, bins, period_min, period_max)
+                  for s in range(NUM_SHAPES)]
+            bt = [_time_batch(s, nsamp, tsamp, bins, period_min, period_max, trials)
+                  for s in range(NUM_SHAPES)]
+        finally:
+            set_shape(None)
+        table["periodogram"][key] = int(np.argmin(pg))
+        table["batch"][key] = int(np.argmin(bt))
+        table["timings_ms"][key] = {"periodogram": [round(v, 3) for v in pg],
+                                    "batch_ffa": [round(v, 3) for v in bt]}
+        log(f"bins {key}: periodogram ms by shape {table['timings_ms'][key]['periodogram']}, "
+            f"batch of {trials} ffa ms by shape {table['timings_ms'][key]['batch_ffa']}")
+    set_shape(None)
+    path = out or tuning_path()
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1)
+    global _cache
+    _cache = None
+    log(f"wrote {path}")
+    return table
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--nsamp", type=int, default=2 ** 22)
+    ap.add_argument("--tsamp", type=float, default=64e-6)
+    ap.add_argument("--period-min", type=float, default=0.1)
+    ap.add_argument("--period-max", type=float, default=10.0)
+    ap.add_argument("--trials", type=int, default=16)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    tune(a.nsamp, a.tsamp, a.period_min, a.period_max, a.trials, out=a.out)
+
+
+if __name__ == "__main__":
+    main()

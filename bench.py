@@ -36,19 +36,25 @@ def log(msg):
 def make_inputs(torch, n_series, nsamp, tsamp, seed, dev):
     """White + red noise (PSD ~ f^-1.5, equal to the white level at 0.1 Hz),
     one series in 16 with a Gaussian pulse train (P log-uniform 0.2-8 s, duty
-    1-5 %, S/N 8-30); generated on the GPU."""
+    1-5 %, S/N 8-30); generated on the GPU.  nsamp: one length for all series
+    or a sequence of per-series lengths."""
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     rs = np.random.RandomState(seed)
-    f = torch.fft.rfftfreq(nsamp, d=tsamp, device=dev)
+    lengths = [int(nsamp)] * n_series if np.isscalar(nsamp) else [int(v) for v in nsamp]
+    # the red noise is shaped at one FFT length (the longest series, rounded
+    # up to a power of two: one FFT plan) and cut to each series' length
+    nfft = 1 << int(np.ceil(np.log2(max(lengths))))
+    f = torch.fft.rfftfreq(nfft, d=tsamp, device=dev)
     shape = torch.zeros_like(f)
     shape[1:] = (f[1:] / 0.1) ** (-0.75)
-    t = torch.arange(nsamp, device=dev, dtype=torch.float64) * tsamp
     xs = []
     for i in range(n_series):
+        nsamp = lengths[i]
+        t = torch.arange(nsamp, device=dev, dtype=torch.float64) * tsamp
         x = torch.randn(nsamp, generator=g, device=dev)
-        w = torch.fft.rfft(torch.randn(nsamp, generator=g, device=dev))
-        x += torch.fft.irfft(w * shape, n=nsamp)
+        w = torch.fft.rfft(torch.randn(nfft, generator=g, device=dev))
+        x += torch.fft.irfft(w * shape, n=nfft)[:nsamp]
         if i % 16 == 0:
             period = float(np.exp(rs.uniform(np.log(0.2), np.log(8.0))))
             duty = rs.uniform(0.01, 0.05)
@@ -149,6 +155,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--seed", type=int, default=20260)
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the heterogeneous batch (lengths log-uniform 2^21-2^23)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain copy kernel over 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -264,6 +272,43 @@ def main():
         plain_tps = nplain / plain_s
         log(f"plain: {nplain} trials in {plain_s:.3f} s -> {plain_tps:.1f} trials/s")
 
+    # heterogeneous batch: 512 series, lengths log-uniform 2^21..2^23 rounded
+    # to a multiple of 1024, same noise model and bins coin
+    c3 = None
+    if not args.no_c3:
+        rs3 = np.random.RandomState(args.seed + 3)
+        lens = (np.exp(rs3.uniform(np.log(2 ** 21), np.log(2 ** 23), size=args.trials))
+                .astype(np.int64) // 1024 * 1024)
+        with torch.cuda.stream(stream):
+            t0 = time.time()
+            x3 = make_inputs(torch, args.trials, lens, tsamp, args.seed + 3, dev)
+            torch.cuda.synchronize()
+            log(f"heterogeneous inputs: {args.trials} series, {int(lens.sum())} samples in "
+                f"{time.time() - t0:.1f} s")
+            b3 = BatchSearch(lens, tsamp, bmin, bmax, pmin, pmax, deredden=True, rmed_width=4.0,
+                             already_normalised=False, group_size=args.group_size)
+            o3 = b3.allocate_outputs()
+            b3.bind(x3, o3)
+            for _ in range(args.warmup):
+                b3.run(use_graph=use_graph)
+            torch.cuda.synchronize()
+            ev0.record(stream)
+            for _ in range(args.steps):
+                b3.run(use_graph=use_graph)
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            b3.check()
+            ms3 = ev0.elapsed_time(ev1) / args.steps
+        c3 = {"workload": (f"{args.trials} series, lengths log-uniform 2^21-2^23 (multiple of "
+                           f"1024, seeded), same noise, bins coin and search as the headline"),
+              "ms_per_batch": round(ms3, 3),
+              "trials_per_s": round(args.trials / (ms3 * 1e-3), 2),
+              "equiv_2p22_trials_per_s": round(float(lens.sum()) / 2 ** 22 / (ms3 * 1e-3), 2),
+              "total_samples": int(lens.sum())}
+        log(f"heterogeneous batch: {c3}")
+        b3.close()
+        del x3, o3, b3
+
     # differentiable search (config 4 shape): forward vs forward + adjoint
     grad = fwd_bwd_timing(torch, _lib)
     log(f"fwd+bwd: {grad}")
@@ -339,6 +384,7 @@ def main():
                     "frac": round(lds_achieved / LDS_PEAK_GBS, 4),
                     "bytes": "8 B per FFA float add"},
         },
+        "heterogeneous_batch": c3,
         "fwd_bwd": grad,
         "cpu_baseline": cpu,
     }

@@ -109,3 +109,27 @@ def test_helpers(lc, oracle):
 def test_benchmark_ffa2(lc):
     t = lc.benchmark_ffa2(64, 256, 3)
     assert 0 < t < 1.0
+
+
+def test_golden_fixtures(lc):
+    """The HIP path against the compiled reference's own outputs
+    (tests/golden/reference_outputs.npz, written by tests/golden/make_golden.py)."""
+    import os
+    G = np.load(os.path.join(os.path.dirname(__file__), "golden", "reference_outputs.npz"))
+    for i in range(5):
+        assert np.array_equal(lc.ffa2(G[f"ffa2_{i}_in"]), G[f"ffa2_{i}_out"])
+    np.testing.assert_allclose(lc.snr2(G["snr2_in"], G["snr2_widths"], 1.3), G["snr2_out"],
+                               rtol=1e-5, atol=1e-4)
+    for i in range(4):
+        f = float(G[f"downsample_{i}_f"])
+        got = lc.downsample(G["downsample_in"], f)
+        np.testing.assert_allclose(got, G[f"downsample_{i}_out"], rtol=0, atol=3e-7 * f + 1e-6)
+    for i in range(2):
+        got = lc.running_median(G["rmed_in"], int(G[f"rmed_{i}_width"]))
+        assert np.array_equal(got, G[f"rmed_{i}_out"])
+    a = G["pgram_args"]
+    P, F, S = lc.periodogram(G["pgram_in"], float(G["pgram_tsamp"]), G["pgram_widths"],
+                             a[0], a[1], int(a[2]), int(a[3]))
+    assert np.array_equal(P, G["pgram_periods"])
+    assert np.array_equal(F, G["pgram_foldbins"])
+    np.testing.assert_allclose(S, G["pgram_snrs"], rtol=1e-5, atol=1e-4)

@@ -327,3 +327,34 @@ def test_normalise_moments(oracle):
     x = (np.random.default_rng(0).standard_normal(100_000) * 5 + 3).astype(np.float32)
     y = oracle.normalise(x)
     assert abs(y.mean()) < 1e-6 and abs(y.std() - 1) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Golden fixtures written from the compiled reference (tests/golden/make_golden.py)
+# ---------------------------------------------------------------------------
+
+def _golden():
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "reference_outputs.npz")
+    return np.load(path)
+
+
+def test_oracle_matches_golden_fixtures():
+    from oracle import oracle as O
+    G = _golden()
+    L = O.oracle()
+    for i in range(5):
+        assert np.array_equal(L.ffa2(G[f"ffa2_{i}_in"]), G[f"ffa2_{i}_out"])
+    assert np.array_equal(L.snr2(G["snr2_in"], G["snr2_widths"], 1.3), G["snr2_out"])
+    for i in range(4):
+        got = L.downsample(G["downsample_in"], float(G[f"downsample_{i}_f"]))
+        assert np.array_equal(got, G[f"downsample_{i}_out"])
+    for i in range(2):
+        got = L.running_median(G["rmed_in"], int(G[f"rmed_{i}_width"]))
+        assert np.array_equal(got, G[f"rmed_{i}_out"])
+    a = G["pgram_args"]
+    P, F, S = L.periodogram(G["pgram_in"], float(G["pgram_tsamp"]), G["pgram_widths"],
+                            a[0], a[1], int(a[2]), int(a[3]))
+    assert np.array_equal(P, G["pgram_periods"])
+    assert np.array_equal(F, G["pgram_foldbins"])
+    assert np.array_equal(S, G["pgram_snrs"])

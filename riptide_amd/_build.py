@@ -8,6 +8,7 @@ import os
 import shutil
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -16,6 +17,10 @@ BUILD = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["ffa_kernels.hip", "series_kernels.hip", "batch.hip", "adjoint.hip", "api.hip"]
 GCC_SOURCES = ["plan.cpp", "ffa_plan.cpp"]
+# ffa_variant.hip is compiled once per FFA kernel variant (kFfaVariants in
+# csrc/tasks.h), each holding the merge code of its own row lengths
+FFA_VARIANT_SOURCE = "ffa_variant.hip"
+FFA_VARIANTS = 10
 
 ARCH = os.environ.get("RIPTIDE_AMD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
@@ -50,21 +55,34 @@ def _run(cmd):
     return r
 
 
+def _jobs():
+    try:
+        return max(1, min(os.cpu_count() or 1, int(os.environ.get("MAX_JOBS", "8"))))
+    except ValueError:
+        return 1
+
+
 def build(verbose=False):
-    """Compiles every source and links the shared library (always rebuilds)."""
+    """Compiles every source (in parallel) and links the shared library
+    (always rebuilds)."""
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    cmds = []
     for src in HIP_SOURCES:
         path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(BUILD, src + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
-        objs.append(obj)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj])
+    for v in range(FFA_VARIANTS):
+        obj = os.path.join(BUILD, f"ffa_variant_{v}.o")
+        cmds.append([HIPCC, *HIP_FLAGS, f"-DRT_FFA_VARIANT={v}", "-c",
+                     os.path.join(CSRC, FFA_VARIANT_SOURCE), "-o", obj])
     for src in GCC_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run(["g++", *GCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        cmds.append(["g++", *GCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    # the slowest compiles (the kernel variants) first
+    cmds.sort(key=lambda c: 0 if FFA_VARIANT_SOURCE in c[-3] else 1)
+    with ThreadPoolExecutor(_jobs()) as pool:
+        list(pool.map(_run, cmds))
+    objs = [c[-1] for c in cmds]
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

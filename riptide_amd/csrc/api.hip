@@ -142,11 +142,15 @@ static std::vector<float> hb_table(int64_t b0, int64_t nb, const uint64_t* w, in
 static int run_passes(const std::vector<std::vector<FfaTask>>& passes, hipStream_t stream,
                       bool sync, int shape)
 {
+    // the tasks of a pass are independent: grouped by kernel variant, one
+    // launch per variant
     std::vector<FfaTask> all;
     std::vector<size_t> first;
     for (const auto& p : passes) {
         first.push_back(all.size());
-        all.insert(all.end(), p.begin(), p.end());
+        std::vector<FfaTask> q = p;
+        sort_by_variant(q);
+        all.insert(all.end(), q.begin(), q.end());
     }
     if (all.empty()) return RT_OK;
     FfaTask* d_tasks = nullptr;
@@ -157,13 +161,14 @@ static int run_passes(const std::vector<std::vector<FfaTask>>& passes, hipStream
     if (st) return st;
     RT_HIP(hipMemsetAsync(errp, 0, sizeof(int), stream));
     for (size_t i = 0; i < passes.size(); ++i)
-        RT_HIP(launch_ffa_tasks(d_tasks + first[i], (int)passes[i].size(), (int*)errp, stream,
-                                shape));
+        RT_HIP(launch_ffa_task_list(d_tasks + first[i], all.data() + first[i],
+                                    (int)passes[i].size(), (int*)errp, stream, shape));
     if (sync) {
         int herr = 0;
         RT_HIP(hipMemcpyAsync(&herr, errp, sizeof(int), hipMemcpyDeviceToHost, stream));
         RT_HIP(hipStreamSynchronize(stream));
-        if (herr) return fail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+        if (herr & 1) return fail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+        if (herr) return fail(RT_EDEVICE, "FFA task launched with the wrong kernel variant");
     }
     return RT_OK;
 }
@@ -355,7 +360,8 @@ int rt_time_ffa(int64_t rows, int64_t cols, int shape, int with_snr, int64_t rep
     for (int64_t r = -1; r < reps; ++r) {
         if (r == 0) RT_HIP(hipEventRecord(a, nullptr));
         for (size_t i = 0; i < passes.size(); ++i)
-            RT_HIP(launch_ffa_tasks(d_tasks + first[i], (int)passes[i].size(), err, nullptr, shape));
+            RT_HIP(launch_ffa_task_list(d_tasks + first[i], all.data() + first[i],
+                                        (int)passes[i].size(), err, nullptr, shape));
     }
     RT_HIP(hipEventRecord(b, nullptr));
     RT_HIP(hipEventSynchronize(b));
@@ -967,7 +973,8 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
             if (st) return st;
             size_t off = 0;
             for (auto& q : passes) {
-                RT_HIP(launch_ffa_tasks(d_tasks + off, (int)q.size(), (int*)errp, s, shape));
+                RT_HIP(launch_ffa_task_list(d_tasks + off, all.data() + off, (int)q.size(),
+                                            (int*)errp, s, shape));
                 off += q.size();
             }
             // dL/d(transform rows): S/N backward on the evaluated rows, zero beyond
@@ -998,7 +1005,8 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
     int herr = 0;
     RT_HIP(hipMemcpyAsync(&herr, errp, sizeof(int), hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
-    if (herr) return fail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+    if (herr & 1) return fail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+    if (herr) return fail(RT_EDEVICE, "FFA task launched with the wrong kernel variant");
     return RT_OK;
 }
 

@@ -49,9 +49,18 @@ __global__ void fill_int_kernel(int* p, int n, int v)
     if (i < n) p[i] = v;
 }
 
+// Transforms of one group that share a kernel variant: one persistent
+// launch, one queue counter.
+struct Run {
+    int tr0 = 0, ntr = 0;          // transforms in the device table
+    int variant = 0;
+    int queue = 0;                 // queue counter index
+};
+
 struct Group {
     int s0 = 0, s1 = 0;            // series [s0, s1)
     int tr0 = 0, ntr = 0;          // transforms in the device table
+    std::vector<Run> runs;         // [tr0, tr0 + ntr) split by kernel variant
     int ds0 = 0;                   // DsSeriesDev entries [ds0, ds0 + s1 - s0)
     int total_chunks = 0;
     int halo = 2;
@@ -100,7 +109,8 @@ struct Batch {
     float* d_prep = nullptr;
     float* d_scratch = nullptr;
     float* d_work = nullptr;             // dereddening + normalisation scratch
-    int* d_queue = nullptr;
+    int* d_queue = nullptr;        // one counter per run
+    int nqueues = 0;
     int* d_err = nullptr;
     int64_t ds_floats = 0, prep_floats = 0, slot_floats = 0, work_floats = 0;
     int slots = 0;
@@ -317,11 +327,24 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                 local_pidx.push_back((int)it);
             }
         }
+        // by kernel variant, then largest first
         std::stable_sort(order.begin(), order.end(),
-                         [](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& c) {
+                         [&local](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& c) {
+                             const int va = ffa_variant(local[a.second].bins);
+                             const int vc = ffa_variant(local[c.second].bins);
+                             if (va != vc) return va < vc;
                              return a.first > c.first;
                          });
         for (const auto& o : order) {
+            const int v = ffa_variant(local[o.second].bins);
+            if (g.runs.empty() || g.runs.back().variant != v) {
+                Run r;
+                r.tr0 = (int)b->host_trs.size();
+                r.variant = v;
+                r.queue = b->nqueues++;
+                g.runs.push_back(r);
+            }
+            ++g.runs.back().ntr;
             b->host_trs.push_back(local[o.second]);
             b->tr_series.push_back(local_series[o.second]);
             b->tr_plan_index.push_back(local_pidx[o.second]);
@@ -380,7 +403,7 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
     BT_HIP(hipMalloc(&b->d_prep, (size_t)b->prep_floats * sizeof(float)));
     BT_HIP(hipMalloc(&b->d_scratch, (size_t)b->slots * b->slot_floats * sizeof(float)));
     BT_HIP(hipMalloc(&b->d_work, (size_t)b->work_floats * sizeof(float)));
-    BT_HIP(hipMalloc(&b->d_queue, std::max<size_t>(b->groups.size(), 1) * sizeof(int)));
+    BT_HIP(hipMalloc(&b->d_queue, std::max<size_t>(b->nqueues, 1) * sizeof(int)));
     BT_HIP(hipMalloc(&b->d_err, sizeof(int)));
     BT_HIP(hipMemset(b->d_err, 0, sizeof(int)));
     *out = b;
@@ -483,11 +506,16 @@ static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
         if (g.total_chunks > 0)
             BT_HIP(launch_downsample_packed(b->d_ds_series + g.s0, g.s1 - g.s0, b->d_ds_steps,
                                             g.total_chunks, kDsChunk, g.halo, s));
-        hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, b->d_queue + gi, 1, 0);
-        BT_HIP(hipGetLastError());
+        if (!g.runs.empty()) {
+            // queue counters of the group's runs are consecutive
+            hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s,
+                               b->d_queue + g.runs[0].queue, (int)g.runs.size(), 0);
+            BT_HIP(hipGetLastError());
+        }
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 2], s));
-        BT_HIP(launch_ffa_batch(b->d_trs + g.tr0, g.ntr, b->d_queue + gi, b->d_scratch,
-                                b->slot_floats, b->slots, b->d_err, s, b->shape));
+        for (const Run& r : g.runs)
+            BT_HIP(launch_ffa_batch(b->d_trs + r.tr0, r.ntr, b->d_queue + r.queue, b->d_scratch,
+                                    b->slot_floats, b->slots, b->d_err, s, b->shape, r.variant));
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 3], s));
     }
     return RT_OK;
@@ -554,7 +582,8 @@ int rt_batch_check(void* handle, void* stream)
     int herr = 0;
     BT_HIP(hipStreamSynchronize((hipStream_t)stream));
     BT_HIP(hipMemcpy(&herr, b->d_err, sizeof(int), hipMemcpyDeviceToHost));
-    if (herr) return bfail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+    if (herr & 1) return bfail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
+    if (herr) return bfail(RT_EDEVICE, "FFA transform launched with the wrong kernel variant");
     return RT_OK;
 }
 

@@ -1,18 +1,8 @@
-// FFA transform + boxcar S/N kernels for gfx950.
-//
-// One workgroup (16 wavefronts) computes one FfaTask: it stages the rows a
-// tile of the merge tree depends on in LDS, applies the tree's merge levels
-// in place (every level is read into registers, then written back after a
-// barrier), and either writes the finished rows back to HBM or turns each
-// finished row into per-width boxcar S/N without the row ever leaving LDS.
-//
-// Merge order and row map follow transforms.hpp:13-50: node of M rows =
-// head (M>>1 rows) + tail; output row s adds head row h(s) and tail row t(s)
-// rolled left by s - t(s), where h and t are computed in float32 as
-// trunc(k*s + 0.5f) with k = (rows_child - 1)/(M - 1).  Every output element
-// is one float32 add of the same two operands as the reference, so the
-// transform is bit-identical.
+// Boxcar S/N and global-memory FFA kernels for gfx950, and the dispatch of
+// the FFA task kernels (ffa_task.h) to their compiled variants.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "ffa_device.h"
 #include "ffa_kernels.h"
@@ -20,936 +10,6 @@
 #include "tasks.h"
 
 namespace rt {
-
-// LDS layout of the FFA task kernel (bytes are dynamic; see ffa_lds_bytes()):
-//   float    zero[bins]                       all -0.0f (tail of a copied row)
-//   float    data[cap_rows * bins]            staged rows
-//   uint32_t tab[kFfaMaxLevels * cap_rows]    packed row map of every level
-//   Seg      seg[2^(kMaxConeLevels+1)]        cone segments (cone tasks only)
-struct Seg {
-    int m;     // node rows (0: empty slot)
-    int r0;    // node first row (transform-relative)
-    int lo;    // first needed row (node-relative)
-    int len;   // number of needed rows
-    int base;  // first LDS row of this segment in its level's layout
-};
-
-__device__ __forceinline__ int seg_slot(int t, int j) { return (1 << t) - 1 + j; }
-
-// Finds the segment slot of level t whose LDS range holds row p.
-__device__ __forceinline__ int seg_find(const Seg* seg, int t, int p)
-{
-    const int first = (1 << t) - 1;
-    int lo = 0, hi = (1 << t) - 1;
-    // last slot with base <= p and len > 0; bases are non-decreasing
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (seg[first + mid].base <= p)
-            lo = mid;
-        else
-            hi = mid - 1;
-    }
-    // empty slots share the base of the next non-empty one; step back to the
-    // slot that really contains p
-    while (lo > 0 && (seg[first + lo].len == 0 || seg[first + lo].base + seg[first + lo].len <= p))
-        --lo;
-    return first + lo;
-}
-
-// Merge coefficients k = (rows_child - 1)/(M - 1) (float32 division, as
-// transforms.hpp:17-18) of every node size of a merge tree: the nodes at
-// relative depth t of an M-row node have (M >> t) or (M >> t) + 1 rows, so
-// two entries per depth; entry 2t + (m - (M >> t)).
-struct MergeCoef {
-    float kh[2 * (kFfaMaxLevels + 1)];
-    float kt[2 * (kFfaMaxLevels + 1)];
-};
-
-__device__ __forceinline__ void fill_merge_coef(MergeCoef* mc, int M, int levels, int lane)
-{
-    if (lane < 2 * (levels + 1)) {
-        const int t = lane >> 1;
-        const int m = (M >> t) + (lane & 1);
-        const int mh = m >> 1;
-        const int mt = m - mh;
-        mc->kh[lane] = m > 1 ? ((float)mh - 1.0f) / ((float)m - 1.0f) : 0.0f;
-        mc->kt[lane] = m > 1 ? ((float)mt - 1.0f) / ((float)m - 1.0f) : 0.0f;
-    }
-}
-
-__device__ __forceinline__ int coef_slot(int M, int t, int m) { return 2 * t + (m - (M >> t)); }
-
-// Builds the cone of a tile top-down with one wavefront (no workgroup
-// barrier): level 0 is the tile itself; level t+1 holds the rows of the
-// children that level t reads.  Writes R_levels (rows at the bottom) to
-// *bottom_rows.  A level has at most 2^kMaxConeLevels slots, two per lane.
-__device__ __forceinline__ void build_cone_wave(Seg* seg, const FfaTask& tk, const MergeCoef* mc,
-                                                int* bottom_rows, int lane)
-{
-    static_assert((2 << (kMaxConeLevels - 1)) <= 2 * kWave, "cone level wider than a wavefront");
-    const int M = tk.node_m;
-    if (lane == 0) {
-        Seg r;
-        r.m = M;
-        r.r0 = tk.node_r0;
-        r.lo = tk.lo;
-        r.len = tk.hi - tk.lo;
-        r.base = 0;
-        seg[0] = r;
-    }
-    int bottom = tk.hi - tk.lo;
-    for (int t = 0; t < tk.levels; ++t) {
-        // the slots of level t were written by other lanes of this wavefront
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        const int j = lane;  // parent slot (2^t <= 64)
-        Seg a, b;
-        a.m = 0; a.r0 = 0; a.lo = 0; a.len = 0; a.base = 0;
-        b = a;
-        if (j < (1 << t)) {
-            const Seg p = seg[seg_slot(t, j)];
-            a.r0 = p.r0;
-            b.r0 = p.r0;
-            if (p.len > 0 && p.m == 1) {
-                a = p;  // single-row leaf: carried down unchanged
-            } else if (p.len > 0 && p.m >= 2) {
-                const int mh = p.m >> 1;
-                const int mt = p.m - mh;
-                const int ks = coef_slot(M, t, p.m);
-                const float kh = mc->kh[ks];
-                const float kt = mc->kt[ks];
-                const int s0 = p.lo, s1 = p.lo + p.len - 1;
-                const int h0 = merge_row(kh, s0), h1 = merge_row(kh, s1);
-                const int t0 = merge_row(kt, s0), t1 = merge_row(kt, s1);
-                a.m = mh; a.r0 = p.r0; a.lo = h0; a.len = h1 - h0 + 1;
-                b.m = mt; b.r0 = p.r0 + mh; b.lo = t0; b.len = t1 - t0 + 1;
-            }
-        }
-        // exclusive scan of the children's lengths, children 2j and 2j+1 on lane j
-        const int run = a.len + b.len;
-        int incl = run;
-        for (int d = 1; d < kWave; d <<= 1) {
-            const int o = __shfl_up(incl, d, kWave);
-            if (lane >= d) incl += o;
-        }
-        a.base = incl - run;
-        b.base = a.base + a.len;
-        if (j < (1 << t)) {
-            seg[seg_slot(t + 1, 2 * j)] = a;
-            seg[seg_slot(t + 1, 2 * j + 1)] = b;
-        }
-        bottom = __shfl(incl, kWave - 1, kWave);
-    }
-    if (lane == 0) *bottom_rows = bottom;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-}
-
-// Packed per-row table entry of one merge level: head row (10 bits), tail
-// row + 1 (10 bits, 0 = copy the head row) and circular shift (12 bits).
-__device__ __forceinline__ uint32_t pack_row(int a, int b, int sh)
-{
-    return (uint32_t)a | ((uint32_t)(b + 1) << 10) | ((uint32_t)sh << 20);
-}
-
-// Row-table entries of every merge level for row p of a whole-node task (the
-// node at relative depth t holding row p reads head row a and tail row b,
-// rolled left by sh), walking the tree down from the root once.
-__device__ __forceinline__ void identity_row_tables(int M, int levels, int p, int bins,
-                                                    const MergeCoef* mc, uint32_t* tab,
-                                                    int cap_rows)
-{
-    int r0 = 0, m = M;
-    for (int t = 0; t < levels; ++t) {
-        uint32_t e;
-        if (m <= 1) {
-            e = pack_row(p, -1, 0);
-        } else {
-            const int mh = m >> 1;
-            const int ks = coef_slot(M, t, m);
-            const int s = p - r0;
-            const int h = merge_row(mc->kh[ks], s);
-            const int tt = merge_row(mc->kt[ks], s);
-            int sh = s - tt;
-            while (sh >= bins) sh -= bins;
-            e = pack_row(r0 + h, r0 + mh + tt, sh);
-            if (s < mh) {
-                m = mh;
-            } else {
-                r0 += mh;
-                m -= mh;
-            }
-        }
-        tab[(size_t)t * cap_rows + p] = e;
-    }
-}
-
-// Row map of one merge level of a cone task (level t reads level t+1).
-__device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int bins, int M,
-                                             const MergeCoef* mc, int* a, int* b, int* sh)
-{
-    const int slot = seg_find(seg, t, p);
-    const Seg g = seg[slot];
-    const int j = slot - ((1 << t) - 1);
-    const Seg ca = seg[seg_slot(t + 1, 2 * j)];
-    const Seg cb = seg[seg_slot(t + 1, 2 * j + 1)];
-    const int s = g.lo + (p - g.base);
-    if (g.m == 1) {
-        *a = ca.base + (s - ca.lo);
-        *b = -1;
-        *sh = 0;
-        return;
-    }
-    const int ks = coef_slot(M, t, g.m);
-    const int h = merge_row(mc->kh[ks], s);
-    const int tt = merge_row(mc->kt[ks], s);
-    *a = ca.base + (h - ca.lo);
-    *b = cb.base + (tt - cb.lo);
-    int r = s - tt;
-    while (r >= bins) r -= bins;
-    *sh = r;
-}
-
-
-// Rows in the layout of merge level t of a task (the rows that level writes).
-__device__ __forceinline__ int level_rows(bool cone, const Seg* seg, int t, int rows)
-{
-    if (!cone) return rows;
-    const int first = seg_slot(t, 0);
-    const int n = 1 << t;
-    return seg[first + n - 1].base + seg[first + n - 1].len;
-}
-
-// 32-bit LDS address of a pointer into LDS, and a load from one.
-typedef __attribute__((address_space(3))) const float lds_cfloat;
-__device__ __forceinline__ uint32_t lds_address(const float* p)
-{
-    return (uint32_t)(uintptr_t)(lds_cfloat*)p;
-}
-__device__ __forceinline__ float lds_load(uint32_t a) { return *(lds_cfloat*)(uintptr_t)a; }
-
-// Everything of a task that depends on the row length, with at most K
-// columns per lane (bins <= 64*K; exactly K when EXACT) known at compile time.
-// LDS rows are `pitch` floats apart (odd, so that lanes reading the same
-// column of consecutive rows hit distinct banks).  Wavefront w owns rows
-// w, w+NW, ...; lane l owns columns l + 64*q.
-template <int NW, int K, int STAGE, bool EXACT>
-struct RowCore {
-    static constexpr int R = ffa_rows_per_wave(K, STAGE);
-    static_assert(R >= 1, "column count beyond the register staging");
-    // rows per branch-free group of the merge loop (about 12 loads in flight)
-    static constexpr int G = K >= 12 ? 1 : K >= 6 ? 2 : K >= 3 ? 4 : 8;
-
-    // Column q of this lane exists in a row of `bins` columns.
-    static __device__ __forceinline__ bool col_ok(int q, int kd, bool last_ok, int lane, int bins)
-    {
-        if (EXACT) return q < K - 1 || last_ok;
-        return q < kd && lane + q * kWave < bins;
-    }
-
-    // Stages `rows` rows into LDS: LDS row p is global row grow[p] (cone
-    // tasks) or r0 + p (whole nodes).  Each wavefront issues the loads of all
-    // its rows before the first store.
-    static __device__ __forceinline__ void load(float* data, gcfloat* src, const uint32_t* grow,
-                                                int r0, int rows, int bins, int pitch, int wave,
-                                                int lane)
-    {
-        const int kd = (bins + kWave - 1) / kWave;
-        const bool last_ok = lane + (K - 1) * kWave < bins;
-        for (int base = 0; base < rows; base += R * NW) {
-            const int nr = (rows - base - wave + NW - 1) / NW;
-            float v[R * K];
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-                if (i < nr) {
-                    const int p = base + wave + i * NW;
-                    const int g = grow ? (int)__builtin_amdgcn_readfirstlane(grow[p]) : r0 + p;
-                    gcfloat* srow = src + (size_t)g * bins + lane;
-#pragma unroll
-                    for (int q = 0; q < K; ++q)
-                        if (col_ok(q, kd, last_ok, lane, bins)) v[i * K + q] = srow[q * kWave];
-                }
-            }
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-                if (i < nr) {
-                    float* drow = data + (base + wave + i * NW) * pitch + lane;
-#pragma unroll
-                    for (int q = 0; q < K; ++q)
-                        if (col_ok(q, kd, last_ok, lane, bins)) drow[q * kWave] = v[i * K + q];
-                }
-            }
-        }
-    }
-
-    // Writes LDS rows 0..rows-1 to consecutive global rows of dst.
-    static __device__ __forceinline__ void store(gfloat* dst, const float* data, int rows,
-                                                 int bins, int pitch, int wave, int lane)
-    {
-        const int kd = (bins + kWave - 1) / kWave;
-        const bool last_ok = lane + (K - 1) * kWave < bins;
-        for (int p = wave; p < rows; p += NW) {
-            const float* srow = data + p * pitch + lane;
-            gfloat* drow = dst + (size_t)p * bins + lane;
-#pragma unroll
-            for (int q = 0; q < K; ++q)
-                if (col_ok(q, kd, last_ok, lane, bins)) drow[q * kWave] = srow[q * kWave];
-        }
-    }
-
-    // All merge levels, bottom-up, in place.  Per element: two LDS reads
-    // (head column c, tail column (c + shift) mod bins, the wrap chosen by
-    // one compare against a per-row scalar) and one add; the results stay in
-    // registers until the barrier, then overwrite the level in place.  Row -1
-    // of `data` is the -0.0f row, the tail of copied rows.
-    static __device__ __forceinline__ int merge(const Seg* seg, float* data, const uint32_t* tab,
-                                                int rows, int levels, bool cone, int bins,
-                                                int pitch, int cap_rows, int wave, int lane)
-    {
-        const int kd = (bins + kWave - 1) / kWave;
-        const bool last_ok = lane + (K - 1) * kWave < bins;
-        for (int t = levels - 1; t >= 0; --t) {
-            const int out_rows = level_rows(cone, seg, t, rows);
-            const uint32_t* tl = tab + (size_t)t * cap_rows;
-            const int nr = (out_rows - wave + NW - 1) / NW;
-            // this wavefront's row-table entries, all read before the first use
-            uint32_t ent[R];
-#pragma unroll
-            for (int i = 0; i < R; ++i) ent[i] = tl[min(wave + i * NW, out_rows - 1)];
-            float v[R * K];
-            // rows in groups of G without a branch between them, so that the
-            // loads of a whole group are in flight together (a group may
-            // run past this wavefront's last row: its entries are clamped to
-            // a real row and its results are never written)
-#pragma unroll
-            for (int i0 = 0; i0 < R; i0 += G) {
-                if (i0 < nr) {
-#pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const int i = i0 + g;
-                        if (i >= R) break;
-                        const uint32_t e = __builtin_amdgcn_readfirstlane(ent[i]);
-                        const int a = (int)(e & 1023u);
-                        const int bt = (int)((e >> 10) & 1023u) - 1;
-                        const int sh = (int)(e >> 20);
-                        const float* hrow = data + a * pitch + lane;
-                        // LDS addresses of the tail column c + sh before and
-                        // after the wrap; the second is hidden from the
-                        // optimiser so that each element costs one compare
-                        // and one select, the column step staying an
-                        // immediate offset
-                        const uint32_t t1 = lds_address(data + bt * pitch + lane + sh);
-                        uint32_t t2 = t1 - bins * 4;
-                        asm volatile("" : "+v"(t2));
-                        const int lim = bins - sh;
-#pragma unroll
-                        for (int q = 0; q < K; ++q) {
-                            if (EXACT || q < kd) {
-                                const uint32_t tb = lane + q * kWave >= lim ? t2 : t1;
-                                const float tv = lds_load(tb + q * kWave * 4);
-                                v[i * K + q] = hrow[q * kWave] + tv;
-                            }
-                        }
-                    }
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int i = 0; i < R; ++i) {
-                if (i < nr) {
-                    float* orow = data + (wave + i * NW) * pitch + lane;
-#pragma unroll
-                    for (int q = 0; q < K; ++q)
-                        if (col_ok(q, kd, last_ok, lane, bins)) orow[q * kWave] = v[i * K + q];
-                }
-            }
-            __syncthreads();
-            rows = out_rows;
-        }
-        return rows;
-    }
-
-    // Load, merge levels and (kind 0) store of one task whose tables are
-    // built; returns the rows of the final level.
-    static __device__ __forceinline__ int run(const FfaTask& tk, const Seg* seg, float* data,
-                                              const uint32_t* tab, const uint32_t* grow, int rows,
-                                              int pitch, int wave, int lane)
-    {
-        const int bins = __builtin_amdgcn_readfirstlane(tk.bins);
-        load(data, as_global(tk.src), grow, __builtin_amdgcn_readfirstlane(tk.node_r0), rows, bins,
-             pitch, wave, lane);
-        __syncthreads();
-        rows = merge(seg, data, tab, rows, __builtin_amdgcn_readfirstlane(tk.levels), tk.cone != 0,
-                     bins, pitch, __builtin_amdgcn_readfirstlane(tk.pad), wave, lane);
-        if (tk.kind == 0) {
-            store(as_global(tk.dst) + ((size_t)tk.node_r0 + tk.lo) * bins, data, rows, bins, pitch,
-                  wave, lane);
-            __syncthreads();
-        }
-        return rows;
-    }
-};
-
-// Dispatch on the columns per lane: exact instantiations up to 16 columns
-// (bins <= 1024), then one per rows-per-wavefront class for wider rows.
-template <int NW, int STAGE>
-__device__ __forceinline__ int run_rows(const FfaTask& tk, const Seg* seg, float* data,
-                                        const uint32_t* tab, const uint32_t* grow, int rows,
-                                        int pitch, int wave, int lane)
-{
-    const int k = (tk.bins + kWave - 1) / kWave;
-#define RT_CORE_ARGS tk, seg, data, tab, grow, rows, pitch, wave, lane
-    switch (k) {
-#define RT_CORE_K(K)                                                                           \
-    case K:                                                                                    \
-        return RowCore<NW, K, STAGE, true>::run(RT_CORE_ARGS);
-    RT_CORE_K(1) RT_CORE_K(2) RT_CORE_K(3) RT_CORE_K(4) RT_CORE_K(5) RT_CORE_K(6)
-    RT_CORE_K(7) RT_CORE_K(8) RT_CORE_K(9) RT_CORE_K(10) RT_CORE_K(11) RT_CORE_K(12)
-    RT_CORE_K(13) RT_CORE_K(14) RT_CORE_K(15) RT_CORE_K(16)
-#undef RT_CORE_K
-    default: break;
-    }
-    constexpr int K4 = STAGE / 4 > 16 ? STAGE / 4 : 16;
-    constexpr int K3 = STAGE / 3 > 16 ? STAGE / 3 : 16;
-    constexpr int K2 = STAGE / 2 > 16 ? STAGE / 2 : 16;
-    if (STAGE / 4 > 16 && k <= STAGE / 4) return RowCore<NW, K4, STAGE, false>::run(RT_CORE_ARGS);
-    if (STAGE / 3 > 16 && k <= STAGE / 3) return RowCore<NW, K3, STAGE, false>::run(RT_CORE_ARGS);
-    if (STAGE / 2 > 16 && k <= STAGE / 2) return RowCore<NW, K2, STAGE, false>::run(RT_CORE_ARGS);
-    return RowCore<NW, STAGE, STAGE, false>::run(RT_CORE_ARGS);
-#undef RT_CORE_ARGS
-}
-
-// Order-preserving map of a float to an int (and back: the map is its own
-// inverse), so that LDS integer max gives the float max.
-__device__ __forceinline__ int ordered_int(float f)
-{
-    const int i = __float_as_int(f);
-    return i ^ ((i >> 31) & 0x7FFFFFFF);
-}
-__device__ __forceinline__ float ordered_float(int k) { return __int_as_float(k ^ ((k >> 31) & 0x7FFFFFFF)); }
-
-// Window maxima over window starts [i0, i1) of one row for NWC widths, all
-// windows inside the row (i1 + wmax <= bins): each prefix value is read once
-// for all widths.  Every lane of a wavefront works on the same columns of a
-// different row, so all branches are uniform.
-template <int NWC>
-__device__ __forceinline__ void window_max_chunk(const float* cps, int i0, int i1,
-                                                 const int (&w)[kSnrMaxWidths],
-                                                 float (&m)[kSnrMaxWidths])
-{
-    const int plain_end = i1;
-    int i = i0;
-    for (; i + 4 <= plain_end; i += 4) {
-        float c[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) c[u] = cps[i + u];
-        // widths four at a time: all loads of a group issued before the
-        // first use (the empty asm keeps the compiler from interleaving
-        // each load with its use, which would wait on every load)
-#pragma unroll
-        for (int j0 = 0; j0 < NWC; j0 += 4) {
-            float q[4][4];
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-                if (j0 + jj < NWC) {
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) q[jj][u] = cps[i + u + w[j0 + jj]];
-                }
-            asm volatile("" ::: "memory");
-#pragma unroll
-            for (int jj = 0; jj < 4; ++jj)
-                if (j0 + jj < NWC)
-                    m[j0 + jj] = fmaxf(m[j0 + jj], fmaxf(fmaxf(q[jj][0] - c[0], q[jj][1] - c[1]),
-                                                         fmaxf(q[jj][2] - c[2], q[jj][3] - c[3])));
-        }
-    }
-    for (; i < plain_end; ++i) {
-        const float c0 = cps[i];
-#pragma unroll
-        for (int j = 0; j < NWC; ++j) m[j] = fmaxf(m[j], cps[i + w[j]] - c0);
-    }
-}
-
-// Per-width boxcar S/N of `nrows` finished rows staged in LDS (row p at
-// data + p*pitch), written to tk.snr.  `scratch` is free LDS.
-//  1. circular prefix sums in place (kernels.hpp:73-90): each row is cut
-//     into C chunks; one thread per (chunk, row), chunk-major so that the
-//     lanes of a wavefront take the same chunk of consecutive rows.  Chunk
-//     totals in float-float go to LDS, then every chunk is rescanned from
-//     the float-float sum of the chunks before it, writing each partial sum
-//     rounded once to float32 -- what the reference's float64 accumulator
-//     gives.
-//  2. window maxima: the same (chunk, row) threads, all widths per thread
-//     (each prefix value read once per width), combined across chunks with
-//     LDS integer max; then one thread per (row, width) writes the S/N.
-template <int NT>
-__device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pitch, int nrows,
-                                         float* scratch, int scratch_floats, int* s_w,
-                                         float* s_hb)
-{
-    constexpr int NW = NT / kWave;
-    const int tid = threadIdx.x;
-    const int lane = tid & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
-    const int bins = tk.bins;
-    const int nw = tk.nw;
-    if (nrows <= 0) return;
-    // (chunk, row) work items: wavefront item wc covers rows 64*(wc % wpc) ..
-    // of chunk wc / wpc; lanes past the last row repeat it and store nothing
-    const int wpc = (nrows + kWave - 1) / kWave;  // wavefronts per chunk
-    int C = max(1, NW / wpc);
-    C = min(C, max(1, bins / 8));                      // chunks of >= 8 columns
-    C = min(C, max(1, scratch_floats / (2 * nrows)));  // chunk totals fit
-    const int L = (bins + C - 1) / C;
-    float* tot = scratch;  // [C][nrows] float-float pairs
-    // widths and (h + b, b) pairs of the transform: loaded now, stored to
-    // LDS after the first prefix pass, so that their latency is hidden
-    const int nwl = min(nw, kSnrMaxWidths);
-    int w_ld = 0;
-    float hb_ld = 0.0f;
-    if (wave == NW - 1) {
-        if (lane < nwl) w_ld = as_global(tk.widths)[lane];
-        if (lane < 2 * nwl) hb_ld = as_global(tk.hb)[lane];
-    }
-    for (int wc = wave; wc < C * wpc; wc += NW) {
-        const int c = wc / wpc;
-        const int pr = (wc - c * wpc) * kWave + lane;
-        const int p = min(pr, nrows - 1);
-        const float* row = data + (size_t)p * pitch;
-        const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
-        float sh = 0.0f, sl = 0.0f;
-        int j = j0;
-        for (; j + 4 <= j1; j += 4) {
-            float x[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = row[j + u];
-            asm volatile("" ::: "memory");
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                float s, e;
-                two_sum(sh, x[u], s, e);
-                sl += e;
-                sh = s + sl;
-                sl -= sh - s;
-            }
-        }
-        for (; j < j1; ++j) {
-            float s, e;
-            two_sum(sh, row[j], s, e);
-            sl += e;
-            sh = s + sl;
-            sl -= sh - s;
-        }
-        if (pr < nrows) {
-            tot[2 * (c * nrows + p)] = sh;
-            tot[2 * (c * nrows + p) + 1] = sl;
-        }
-    }
-    if (wave == NW - 1) {
-        if (lane < nwl) s_w[lane] = w_ld;
-        if (lane < 2 * nwl) s_hb[lane] = hb_ld;
-    }
-    __syncthreads();
-    for (int wc = wave; wc < C * wpc; wc += NW) {
-        const int c = wc / wpc;
-        const int pr = (wc - c * wpc) * kWave + lane;
-        const int p = min(pr, nrows - 1);
-        float* row = data + (size_t)p * pitch;
-        float sh = 0.0f, sl = 0.0f;
-        for (int cc = 0; cc < c; ++cc)
-            ff_add(sh, sl, tot[2 * (cc * nrows + p)], tot[2 * (cc * nrows + p) + 1], sh, sl);
-        const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
-        const bool own = pr < nrows;
-        int j = j0;
-        for (; j + 4 <= j1; j += 4) {
-            float x[4];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) x[u] = row[j + u];
-            asm volatile("" ::: "memory");
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                float s, e;
-                two_sum(sh, x[u], s, e);
-                sl += e;
-                sh = s + sl;  // rounded partial sum
-                sl -= sh - s;
-                x[u] = sh;
-            }
-            if (own) {
-#pragma unroll
-                for (int u = 0; u < 4; ++u) row[j + u] = x[u];
-            }
-        }
-        for (; j < j1; ++j) {
-            float s, e;
-            two_sum(sh, row[j], s, e);
-            sl += e;
-            sh = s + sl;  // rounded partial sum
-            sl -= sh - s;
-            if (own) row[j] = sh;
-        }
-    }
-    __syncthreads();
-
-    gcfloat* hb = as_global(tk.hb);
-    gcint* gw = as_global(tk.widths);
-    int* mx = reinterpret_cast<int*>(scratch);  // [nrows][nw] window maxima
-    if (nw >= 1 && nw <= kSnrMaxWidths && nrows * nw <= scratch_floats) {
-        for (int e = tid; e < nrows * nw; e += NT) mx[e] = ordered_int(-INFINITY);
-        // widths and (h + b, b) pairs were copied to LDS at the start of
-        // the task
-        int w[kSnrMaxWidths];
-#pragma unroll
-        for (int j = 0; j < kSnrMaxWidths; ++j)
-            w[j] = __builtin_amdgcn_readfirstlane(s_w[j < nw ? j : nw - 1]);
-        const int wmax = __builtin_amdgcn_readfirstlane(s_w[nw - 1]);
-        __syncthreads();
-        // (a) window starts [0, bins - wmax), whose windows all stay inside
-        // the row, in C chunks of equal cost, all widths per thread
-        const int P = max(0, bins - wmax);
-        const int Lw = (P + C - 1) / C;
-        for (int wc = wave; wc < C * wpc; wc += NW) {
-            const int c = wc / wpc;
-            const int pr = (wc - c * wpc) * kWave + lane;
-            const int p = min(pr, nrows - 1);
-            const float* cps = data + (size_t)p * pitch;
-            const int i0 = min(P, c * Lw), i1 = min(P, i0 + Lw);
-            if (i0 >= i1) continue;
-            float m[kSnrMaxWidths];
-#pragma unroll
-            for (int j = 0; j < kSnrMaxWidths; ++j) m[j] = -INFINITY;
-            switch (nw) {
-#define RT_WIN(N) \
-    case N: window_max_chunk<N>(cps, i0, i1, w, m); break;
-                RT_WIN(1) RT_WIN(2) RT_WIN(3) RT_WIN(4) RT_WIN(5) RT_WIN(6) RT_WIN(7) RT_WIN(8)
-                RT_WIN(9) RT_WIN(10) RT_WIN(11) RT_WIN(12) RT_WIN(13) RT_WIN(14) RT_WIN(15)
-                RT_WIN(16)
-#undef RT_WIN
-                default: break;
-            }
-            if (pr < nrows) {
-#pragma unroll
-                for (int j = 0; j < kSnrMaxWidths; ++j)
-                    if (j < nw) atomicMax(&mx[p * nw + j], ordered_int(m[j]));
-            }
-        }
-        // (b) the last wmax window starts, one thread per (row, width),
-        // width-major: windows inside the row, then the ones running past its
-        // end over the extension cps[j] + total
-        for (int job = tid; job < nrows * nw; job += NT) {
-            const int iw = job / nrows;
-            const int p = job - iw * nrows;
-            const float* cps = data + (size_t)p * pitch;
-            const int wj = s_w[iw];
-            const float total = cps[bins - 1];
-            const int split = bins - wj;
-            float d0 = -INFINITY, d1 = -INFINITY;
-            int k = P;
-            for (; k + 4 <= split; k += 4) {
-                float a[4], b[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    a[u] = cps[k + u];
-                    b[u] = cps[k + u + wj];
-                }
-                asm volatile("" ::: "memory");
-                d0 = fmaxf(d0, fmaxf(fmaxf(b[0] - a[0], b[1] - a[1]), fmaxf(b[2] - a[2], b[3] - a[3])));
-            }
-            for (; k < split; ++k) d0 = fmaxf(d0, cps[k + wj] - cps[k]);
-            for (; k + 4 <= bins; k += 4) {
-                float a[4], b[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    a[u] = cps[k + u];
-                    b[u] = cps[k + u - split];
-                }
-                asm volatile("" ::: "memory");
-#pragma unroll
-                for (int u = 0; u < 4; ++u) d1 = fmaxf(d1, (b[u] + total) - a[u]);
-            }
-            for (; k < bins; ++k) d1 = fmaxf(d1, (cps[k - split] + total) - cps[k]);
-            atomicMax(&mx[p * nw + iw], ordered_int(fmaxf(d0, d1)));
-        }
-        __syncthreads();
-        for (int e = tid; e < nrows * nw; e += NT) {
-            const int p = e / nw;
-            const int iw = e - p * nw;
-            const float total = data[(size_t)p * pitch + bins - 1];
-            const float dm = ordered_float(mx[e]);
-            const float snr =
-                __fmaf_rn(dm, s_hb[2 * iw], -(s_hb[2 * iw + 1] * total)) / tk.stdnoise;
-            as_global(tk.snr)[(size_t)(tk.lo + p) * nw + iw] = snr;
-        }
-        __syncthreads();
-        return;
-    }
-    // many widths: one thread per (row, width), width-major (uniform loop
-    // bounds; the odd row pitch keeps the lanes' LDS reads on distinct banks)
-    for (int job = tid; job < nrows * nw; job += NT) {
-        const int iw = job / nrows;
-        const int p = job - iw * nrows;
-        const float* cps = data + (size_t)p * pitch;
-        const int w = gw[iw];
-        const float total = cps[bins - 1];
-        const int split = bins - w;
-        float d0 = -INFINITY, d1 = -INFINITY;
-        int i = 0;
-        for (; i + 2 <= split; i += 2) {
-            d0 = fmaxf(d0, cps[i + w] - cps[i]);
-            d1 = fmaxf(d1, cps[i + 1 + w] - cps[i + 1]);
-        }
-        for (; i < split; ++i) d0 = fmaxf(d0, cps[i + w] - cps[i]);
-        // windows crossing the end: the extension cps[j] + total
-        for (; i < bins; ++i) d1 = fmaxf(d1, (cps[i - split] + total) - cps[i]);
-        const float dm = fmaxf(d0, d1);
-        const float snr = __fmaf_rn(dm, hb[2 * iw], -(hb[2 * iw + 1] * total)) / tk.stdnoise;
-        as_global(tk.snr)[(size_t)(tk.lo + p) * nw + iw] = snr;
-    }
-    __syncthreads();
-}
-
-// Runs one FfaTask with the whole workgroup; LDS contents are dead on return.
-//
-// The row tables of all merge levels are built first (one barrier), then the
-// rows are staged, merged level by level and either written back to HBM or
-// turned into per-width boxcar S/N without leaving LDS.
-template <int NT, int STAGE>
-__device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
-{
-    constexpr int NW = NT / kWave;
-    // task fields are uniform: keep them in scalar registers
-    const int bins = __builtin_amdgcn_readfirstlane(tk.bins);
-    const int pitch = ffa_pitch(bins);
-    const int tid = threadIdx.x;
-    const int lane = tid & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
-    const int cap_rows = __builtin_amdgcn_readfirstlane(tk.pad);
-
-    // row -1 of data is all -0.0f; rows 0..cap_rows-1 hold the staged rows
-    float* data = lds + pitch;
-    uint32_t* tab = reinterpret_cast<uint32_t*>(data + (size_t)cap_rows * pitch);
-    Seg* seg = reinterpret_cast<Seg*>(tab + (size_t)cap_rows * kFfaMaxLevels);
-    __shared__ int s_bottom;
-
-    __shared__ MergeCoef s_coef;
-    const int levels = __builtin_amdgcn_readfirstlane(tk.levels);
-    const int M = __builtin_amdgcn_readfirstlane(tk.node_m);
-    if (levels > kFfaMaxLevels || (tk.cone && levels > kMaxConeLevels) ||
-        (!tk.cone && M > cap_rows)) {
-        // the host sizes tasks so this cannot happen; never overrun LDS
-        if (tid == 0) atomicOr(err, 1);
-        return;
-    }
-
-    for (int c = tid; c < bins; c += NT) lds[c] = -0.0f;
-
-    __shared__ int s_w[kSnrMaxWidths];
-    __shared__ float s_hb[2 * kSnrMaxWidths];
-
-    // ---- merge coefficients, the cone (one wavefront), then the row tables
-    // of every level
-    if (wave == 0) {
-        fill_merge_coef(&s_coef, M, levels, lane);
-        if (tk.cone) {
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            build_cone_wave(seg, tk, &s_coef, &s_bottom, lane);
-        }
-    }
-    __syncthreads();
-    int rows;  // rows in the current level's layout
-    const uint32_t* grow = nullptr;
-    if (tk.cone) {
-        rows = (levels > 0) ? s_bottom : (tk.hi - tk.lo);
-        if (rows > cap_rows) {
-            if (tid == 0) atomicOr(err, 1);
-            __syncthreads();
-            return;
-        }
-        // global source row of every bottom row (in the table slot of the
-        // deepest level, which no merge level uses), then every level's map
-        uint32_t* gr = tab + (size_t)(kFfaMaxLevels - 1) * cap_rows;
-        for (int p = tid; p < rows; p += NT) {
-            const Seg g = seg[seg_find(seg, levels, p)];
-            gr[p] = (uint32_t)(g.r0 + g.lo + (p - g.base));
-        }
-        // one flat loop over (level, row) so that every thread has work
-        for (int e = tid; e < levels * cap_rows; e += NT) {
-            const int t = e / cap_rows;
-            const int p = e - t * cap_rows;
-            if (p < level_rows(true, seg, t, 0)) {
-                int a, b, sh;
-                cone_row_map(seg, t, p, bins, M, &s_coef, &a, &b, &sh);
-                tab[e] = pack_row(a, b, sh);
-            }
-        }
-        grow = gr;
-    } else {
-        rows = M;
-        for (int p = tid; p < rows; p += NT)
-            identity_row_tables(M, levels, p, bins, &s_coef, tab, cap_rows);
-    }
-    __syncthreads();
-
-    // ---- staging, merge levels (and the store of kind-0 tasks)
-    rows = run_rows<NW, STAGE>(tk, seg, data, tab, grow, rows, pitch, wave, lane);
-    if (tk.kind == 0) return;
-
-    const int s_end = min(tk.hi, tk.rows_eval);
-    const int nrows = max(0, s_end - tk.lo);
-    // LDS past the S/N rows is free: the other staged rows, the row tables
-    // and the cone segments
-    float* scratch = data + (size_t)nrows * pitch;
-    const int scratch_floats =
-        (cap_rows - nrows) * pitch + kFfaMaxLevels * cap_rows + (2 << kMaxConeLevels) * 5;
-    snr_rows<NT>(tk, data, pitch, nrows, scratch, scratch_floats, s_w, s_hb);
-}
-
-// One workgroup per task (the task list of one pass).
-template <int NT, int STAGE, int WPS>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) void ffa_task_kernel(
-    const FfaTask* __restrict__ tasks, int ntasks, int* err)
-{
-    extern __shared__ float lds[];
-    if ((int)blockIdx.x >= ntasks) return;
-    const FfaTask tk = tasks[blockIdx.x];
-    run_ffa_task<NT, STAGE>(tk, lds, err);
-}
-
-// Node i of depth d of an m-row merge tree (row order); false when that slot
-// lies under a leaf reached above depth d.
-__device__ __forceinline__ bool tree_node(int m, int d, int i, int* r0, int* nm)
-{
-    int a = 0, M = m;
-    for (int bit = d - 1; bit >= 0; --bit) {
-        if (M <= 1) {
-            if (i & ((1 << (bit + 1)) - 1)) return false;
-            break;
-        }
-        const int mh = M >> 1;
-        if ((i >> bit) & 1) {
-            a += mh;
-            M -= mh;
-        } else {
-            M = mh;
-        }
-    }
-    *r0 = a;
-    *nm = M;
-    return true;
-}
-
-// Next work item of a transform for the persistent kernel, in pass order:
-// pass 0 = whole nodes of depth depth[0] into buf_a, pass p >= 1 = cone tiles
-// of the nodes of depth depth[p], alternating buf_a -> buf_b -> buf_a ...,
-// the last pass producing S/N.  (p, i, lo) is the cursor.
-__device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int32_t* depth,
-                                              int& p, int& i, int& lo, float* buf_a,
-                                              float* buf_b, FfaTask& tk)
-{
-    for (;;) {
-        if (p >= tr.npass) return false;
-        if (tr.npass == 1) {
-            tk.src = tr.src;
-            tk.node_r0 = 0;
-            tk.node_m = tr.rows;
-            tk.lo = 0;
-            tk.hi = tr.rows;
-            tk.levels = tr.levels0;
-            tk.cone = 0;
-            tk.kind = 1;
-            p = 1;
-            return true;
-        }
-        const int d = depth[p];
-        if (i >= (1 << d)) {
-            ++p;
-            i = 0;
-            lo = 0;
-            continue;
-        }
-        int r0, nm;
-        if (!tree_node(tr.rows, d, i, &r0, &nm)) {
-            ++i;
-            lo = 0;
-            continue;
-        }
-        if (p == 0) {
-            int h = 0;
-            while ((1 << h) < nm) ++h;
-            tk.src = tr.src;
-            tk.dst = buf_a;
-            tk.node_r0 = r0;
-            tk.node_m = nm;
-            tk.lo = 0;
-            tk.hi = nm;
-            tk.levels = h;
-            tk.cone = 0;
-            tk.kind = 0;
-            ++i;
-            return true;
-        }
-        const int L = depth[p - 1] - d;
-        const int K = tr.cap - 2 * ((1 << L) - 1);
-        const bool last = (p == tr.npass - 1);
-        if (lo >= nm || (last && lo >= tr.rows_eval)) {
-            ++i;
-            lo = 0;
-            continue;
-        }
-        tk.src = (p & 1) ? buf_a : buf_b;
-        tk.dst = (p & 1) ? buf_b : buf_a;
-        tk.node_r0 = r0;
-        tk.node_m = nm;
-        tk.lo = lo;
-        tk.hi = min(lo + K, nm);
-        tk.levels = L;
-        tk.cone = 1;
-        tk.kind = last ? 1 : 0;
-        lo += K;
-        return true;
-    }
-}
-
-// Persistent batch kernel: every workgroup drains whole transforms from a
-// device queue (largest first) and runs all of a transform's passes itself,
-// its intermediate states in a private scratch slot, so no workgroup ever
-// waits on another.  Rows written by one pass are read by the next only by
-// this workgroup, ordered by the barrier that ends every task.
-template <int NT, int STAGE, int WPS>
-__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) void ffa_batch_kernel(
-    const FfaTransform* __restrict__ trs, int ntr, int* __restrict__ queue, float* scratch,
-    int64_t slot_floats, int* err)
-{
-    extern __shared__ float lds[];
-    __shared__ int s_item;
-    float* buf_a = scratch + (size_t)blockIdx.x * slot_floats;
-    float* buf_b = buf_a + slot_floats / 2;
-    for (;;) {
-        if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
-        __syncthreads();
-        // uniform across the workgroup: keep the descriptor in scalar registers
-        const int item = __builtin_amdgcn_readfirstlane(s_item);
-        __syncthreads();
-        if (item >= ntr) return;
-        const FfaTransform tr = trs[item];
-        FfaTask tk = {};
-        tk.bins = tr.bins;
-        tk.widths = tr.widths;
-        tk.hb = tr.hb;
-        tk.nw = tr.nw;
-        tk.stdnoise = tr.stdnoise;
-        tk.rows_eval = tr.rows_eval;
-        tk.snr = tr.snr;
-        tk.pad = tr.cap;
-        int p = 0, i = 0, lo = 0;
-        // the pass depths are indexed dynamically: read them from memory, so
-        // that the descriptor itself stays in registers
-        const int32_t* depth = trs[item].depth;
-        while (next_ffa_task(tr, depth, p, i, lo, buf_a, buf_b, tk))
-            run_ffa_task<NT, STAGE>(tk, lds, err);
-    }
-}
 
 // Plain S/N of rows x cols profiles (python_bindings.cpp:121-160): one
 // wavefront per row, the row staged in LDS.
@@ -1034,69 +94,60 @@ __global__ void ffa_level_global_kernel(const float* __restrict__ src, float* __
 // Host launchers
 // ---------------------------------------------------------------------------
 
-template <int NT, int STAGE, int WPS>
-static hipError_t launch_tasks_shape(const FfaTask* d_tasks, int ntasks, int* d_err,
-                                     hipStream_t stream, int lds)
-{
-    static bool configured = false;
-    if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void*)ffa_task_kernel<NT, STAGE, WPS>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
-        if (e != hipSuccess) return e;
-        configured = true;
-    }
-    hipLaunchKernelGGL((ffa_task_kernel<NT, STAGE, WPS>), dim3(ntasks), dim3(NT), lds, stream,
-                       d_tasks, ntasks, d_err);
-    return hipGetLastError();
-}
+// ---------------------------------------------------------------------------
+// Dispatch of the FFA task kernels to their variants
+// ---------------------------------------------------------------------------
 
 hipError_t launch_ffa_tasks(const FfaTask* d_tasks, int ntasks, int* d_err, hipStream_t stream,
-                            int shape)
+                            int shape, int variant)
 {
     if (ntasks <= 0) return hipSuccess;
-    const int lds = ffa_lds_bytes(shape);
-    switch (shape) {
-    case 0: return launch_tasks_shape<1024, 40, 4>(d_tasks, ntasks, d_err, stream, lds);
-    case 1: return launch_tasks_shape<512, 40, 4>(d_tasks, ntasks, d_err, stream, lds);
-    default: return launch_tasks_shape<512, 72, 2>(d_tasks, ntasks, d_err, stream, lds);
+    switch (variant) {
+#define RT_V(V) \
+    case V: return launch_ffa_tasks_variant<V>(d_tasks, ntasks, d_err, stream, shape);
+    RT_V(0) RT_V(1) RT_V(2) RT_V(3) RT_V(4) RT_V(5) RT_V(6) RT_V(7) RT_V(8) RT_V(9)
+#undef RT_V
+    default: return hipErrorInvalidValue;
     }
 }
 
-template <int NT, int STAGE, int WPS>
-static hipError_t launch_batch_shape(const FfaTransform* d_trs, int ntr, int* d_queue,
-                                     float* d_scratch, int64_t slot_floats, int slots, int* d_err,
-                                     hipStream_t stream, int lds)
+hipError_t launch_ffa_task_list(const FfaTask* d_tasks, const FfaTask* h_tasks, int ntasks,
+                                int* d_err, hipStream_t stream, int shape)
 {
-    static bool configured = false;
-    if (!configured) {
-        hipError_t e = hipFuncSetAttribute((const void*)ffa_batch_kernel<NT, STAGE, WPS>,
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, lds);
+    for (int i = 0; i < ntasks;) {
+        const int v = ffa_variant(h_tasks[i].bins);
+        int j = i + 1;
+        while (j < ntasks && ffa_variant(h_tasks[j].bins) == v) ++j;
+        const hipError_t e = launch_ffa_tasks(d_tasks + i, j - i, d_err, stream, shape, v);
         if (e != hipSuccess) return e;
-        configured = true;
+        i = j;
     }
-    hipLaunchKernelGGL((ffa_batch_kernel<NT, STAGE, WPS>), dim3(slots), dim3(NT), lds, stream,
-                       d_trs, ntr, d_queue, d_scratch, slot_floats, d_err);
-    return hipGetLastError();
+    return hipSuccess;
+}
+
+void sort_by_variant(std::vector<FfaTask>& tasks)
+{
+    std::stable_sort(tasks.begin(), tasks.end(), [](const FfaTask& a, const FfaTask& b) {
+        return ffa_variant(a.bins) < ffa_variant(b.bins);
+    });
 }
 
 hipError_t launch_ffa_batch(const FfaTransform* d_trs, int ntr, int* d_queue, float* d_scratch,
                             int64_t slot_floats, int slots, int* d_err, hipStream_t stream,
-                            int shape)
+                            int shape, int variant)
 {
     if (ntr <= 0) return hipSuccess;
-    const int lds = ffa_lds_bytes(shape);
-    switch (shape) {
-    case 0:
-        return launch_batch_shape<1024, 40, 4>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots,
-                                               d_err, stream, lds);
-    case 1:
-        return launch_batch_shape<512, 40, 4>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots,
-                                              d_err, stream, lds);
-    default:
-        return launch_batch_shape<512, 72, 2>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots,
-                                              d_err, stream, lds);
+    switch (variant) {
+#define RT_V(V)                                                                              \
+    case V:                                                                                  \
+        return launch_ffa_batch_variant<V>(d_trs, ntr, d_queue, d_scratch, slot_floats, slots, \
+                                           d_err, stream, shape);
+    RT_V(0) RT_V(1) RT_V(2) RT_V(3) RT_V(4) RT_V(5) RT_V(6) RT_V(7) RT_V(8) RT_V(9)
+#undef RT_V
+    default: return hipErrorInvalidValue;
     }
 }
+static_assert(kFfaVariants == 10, "dispatch covers variants 0..9");
 
 hipError_t launch_snr_rows(const float* x, int64_t rows, int cols, const int32_t* widths,
                            const float* hb, int nw, float stdnoise, float* out,

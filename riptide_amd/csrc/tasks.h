@@ -50,6 +50,20 @@ constexpr int kFfaMaxCap = 1023;
 constexpr int kFfaThreads = 1024;
 // Wavefront width on CDNA.
 constexpr int kWave = 64;
+// The FFA task kernels are compiled once per variant: variant v <
+// kFfaWideVariant holds the merge code of k = 2v and 2v+1 columns per lane
+// (k = ceil(bins/64) <= kFfaMaxExactK), variant kFfaWideVariant the classes
+// of wider rows.  Each variant holds little code, so the register allocation
+// of its loops is not spoiled by the code of other row lengths; the host
+// launches work grouped by variant.
+constexpr int kFfaMaxExactK = 17;
+constexpr int kFfaWideVariant = (kFfaMaxExactK + 1) / 2;
+constexpr int kFfaVariants = kFfaWideVariant + 1;
+constexpr int ffa_variant(int bins)
+{
+    return (bins + kWave - 1) / kWave > kFfaMaxExactK ? kFfaWideVariant
+                                                      : ((bins + kWave - 1) / kWave) / 2;
+}
 // Largest merge-level count of a cone task (2^kMaxConeLevels bottom segments).
 constexpr int kMaxConeLevels = 7;
 // Register-resident S/N positions per lane of the standalone S/N kernel

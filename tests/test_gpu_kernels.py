@@ -51,6 +51,27 @@ def test_ffa2_workgroup_shapes(lc, oracle, wg_shape, shape):
     assert np.array_equal(got, oracle.oracle().ffa2(x))
 
 
+# Row lengths of every FFA kernel variant (columns per lane k = ceil(bins/64):
+# variants hold k in {2v, 2v+1} up to k = 17, then one variant for wider rows),
+# with more rows than one workgroup tiles so that cone passes run too.
+VARIANT_BINS = [64, 100, 190, 300, 400, 470, 600, 700, 800, 900, 1024, 1050, 1100, 1500, 2500]
+
+
+@pytest.mark.parametrize("wg_shape", [0, 1, 2])
+@pytest.mark.parametrize("bins", VARIANT_BINS)
+def test_ffa2_every_kernel_variant(lc, oracle, wg_shape, bins):
+    from riptide_amd import _lib
+    rows = max(60, 120000 // bins)
+    rng = np.random.default_rng(bins * 3 + wg_shape)
+    x = rng.standard_normal((rows, bins)).astype(np.float32)
+    _lib.call("rt_set_ffa_shape", wg_shape)
+    try:
+        got = lc.ffa2(x)
+    finally:
+        _lib.call("rt_set_ffa_shape", -1)
+    assert np.array_equal(got, oracle.oracle().ffa2(x))
+
+
 def test_ffa2_kat_8x8(lc):
     x = np.zeros((8, 8), np.float32)
     x[:, 7] = 1

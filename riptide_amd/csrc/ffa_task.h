@@ -250,37 +250,42 @@ struct RowCore {
         return q < kd && lane + q * kWave < bins;
     }
 
-    // Stages `rows` rows into LDS: LDS row p is global row grow[p] (cone
-    // tasks) or r0 + p (whole nodes).  Each wavefront issues the loads of all
-    // its rows before the first store.
-    static __device__ __forceinline__ void load(float* data, gcfloat* src, const uint32_t* grow,
-                                                int r0, int rows, int bins, int pitch, int wave,
-                                                int lane)
+    // Issues the global loads of this wavefront's rows (LDS rows wave + i*NW,
+    // i < nr <= R) into registers: LDS row wave + i*NW is global row
+    // readlane(grow_v, i) (cone tasks; lane i of this wavefront holds it) or
+    // r0 + wave + i*NW (whole nodes).
+    static __device__ __forceinline__ void load_issue(float (&v)[R * K], gcfloat* src, bool cone,
+                                                      uint32_t grow_v, int r0, int nr, int bins,
+                                                      int wave, int lane)
     {
         const int kd = (bins + kWave - 1) / kWave;
         const bool last_ok = lane + (K - 1) * kWave < bins;
-        for (int base = 0; base < rows; base += R * NW) {
-            const int nr = (rows - base - wave + NW - 1) / NW;
-            float v[R * K];
 #pragma unroll
-            for (int i = 0; i < R; ++i) {
-                if (i < nr) {
-                    const int p = base + wave + i * NW;
-                    const int g = grow ? (int)__builtin_amdgcn_readfirstlane(grow[p]) : r0 + p;
-                    gcfloat* srow = src + (size_t)g * bins + lane;
+        for (int i = 0; i < R; ++i) {
+            if (i < nr) {
+                const int g = cone ? (int)__builtin_amdgcn_readlane(grow_v, i) : r0 + wave + i * NW;
+                gcfloat* srow = src + (size_t)g * bins + lane;
 #pragma unroll
-                    for (int q = 0; q < K; ++q)
-                        if (col_ok(q, kd, last_ok, lane, bins)) v[i * K + q] = srow[q * kWave];
-                }
+                for (int q = 0; q < K; ++q)
+                    if (col_ok(q, kd, last_ok, lane, bins)) v[i * K + q] = srow[q * kWave];
             }
+        }
+    }
+
+    // Stores the registers of load_issue to the LDS rows.
+    static __device__ __forceinline__ void load_commit(const float (&v)[R * K], float* data,
+                                                       int nr, int bins, int pitch, int wave,
+                                                       int lane)
+    {
+        const int kd = (bins + kWave - 1) / kWave;
+        const bool last_ok = lane + (K - 1) * kWave < bins;
 #pragma unroll
-            for (int i = 0; i < R; ++i) {
-                if (i < nr) {
-                    float* drow = data + (base + wave + i * NW) * pitch + lane;
+        for (int i = 0; i < R; ++i) {
+            if (i < nr) {
+                float* drow = data + (wave + i * NW) * pitch + lane;
 #pragma unroll
-                    for (int q = 0; q < K; ++q)
-                        if (col_ok(q, kd, last_ok, lane, bins)) drow[q * kWave] = v[i * K + q];
-                }
+                for (int q = 0; q < K; ++q)
+                    if (col_ok(q, kd, last_ok, lane, bins)) drow[q * kWave] = v[i * K + q];
             }
         }
     }
@@ -372,15 +377,23 @@ struct RowCore {
         return rows;
     }
 
-    // Load, merge levels and (kind 0) store of one task whose tables are
-    // built; returns the rows of the final level.
+    // Staging, merge levels and (kind 0) store of one task; returns the rows
+    // of the final level, or -1 if the rows exceed the register staging
+    // (tasks are sized so that they do not).  The row tables are built by
+    // `tables` while this wavefront's row loads are in flight.
+    template <class Tables>
     static __device__ __forceinline__ int run(const FfaTask& tk, const Seg* seg, float* data,
-                                              const uint32_t* tab, const uint32_t* grow, int rows,
-                                              int pitch, int wave, int lane)
+                                              const uint32_t* tab, uint32_t grow_v, int rows,
+                                              int pitch, int wave, int lane, Tables&& tables)
     {
         const int bins = __builtin_amdgcn_readfirstlane(tk.bins);
-        load(data, as_global(tk.src), grow, __builtin_amdgcn_readfirstlane(tk.node_r0), rows, bins,
-             pitch, wave, lane);
+        if (rows > R * NW) return -1;
+        const int nr = (rows - wave + NW - 1) / NW;
+        float v[R * K];
+        load_issue(v, as_global(tk.src), tk.cone != 0, grow_v,
+                   __builtin_amdgcn_readfirstlane(tk.node_r0), nr, bins, wave, lane);
+        tables();
+        load_commit(v, data, nr, bins, pitch, wave, lane);
         __syncthreads();
         rows = merge(seg, data, tab, rows, __builtin_amdgcn_readfirstlane(tk.levels), tk.cone != 0,
                      bins, pitch, __builtin_amdgcn_readfirstlane(tk.pad), wave, lane);
@@ -398,13 +411,13 @@ struct RowCore {
 // k = 2V and 2V+1; the wide variant one instantiation per rows-per-wavefront
 // class for k > kFfaMaxExactK.  Returns -1 for a k the variant does not hold
 // (the host groups work by variant, so this does not happen).
-template <int NW, int STAGE, int V>
+template <int NW, int STAGE, int V, class Tables>
 __device__ __forceinline__ int run_rows(const FfaTask& tk, const Seg* seg, float* data,
-                                        const uint32_t* tab, const uint32_t* grow, int rows,
-                                        int pitch, int wave, int lane)
+                                        const uint32_t* tab, uint32_t grow_v, int rows,
+                                        int pitch, int wave, int lane, Tables&& tables)
 {
     const int k = (tk.bins + kWave - 1) / kWave;
-#define RT_CORE_ARGS tk, seg, data, tab, grow, rows, pitch, wave, lane
+#define RT_CORE_ARGS tk, seg, data, tab, grow_v, rows, pitch, wave, lane, tables
     if constexpr (V < kFfaWideVariant) {
         constexpr int KA = 2 * V, KB = 2 * V + 1;
         static_assert(KB <= kFfaMaxExactK, "variant beyond the exact column counts");
@@ -718,8 +731,10 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
 
 // Runs one FfaTask with the whole workgroup; LDS contents are dead on return.
 //
-// The row tables of all merge levels are built first (one barrier), then the
-// rows are staged, merged level by level and either written back to HBM or
+// Wavefront 0 computes the merge coefficients and the cone (one barrier);
+// every wavefront then issues the global loads of its rows, the row tables of
+// all merge levels are built while they are in flight, the rows are stored to
+// LDS (one barrier), merged level by level and either written back to HBM or
 // turned into per-width boxcar S/N without leaving LDS.
 template <int NT, int STAGE, int V>
 __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int* err)
@@ -754,8 +769,7 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     __shared__ int s_w[kSnrMaxWidths];
     __shared__ float s_hb[2 * kSnrMaxWidths];
 
-    // ---- merge coefficients, the cone (one wavefront), then the row tables
-    // of every level
+    // ---- merge coefficients and the cone (one wavefront)
     if (wave == 0) {
         fill_merge_coef(&s_coef, M, levels, lane);
         if (tk.cone) {
@@ -766,7 +780,7 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     }
     __syncthreads();
     int rows;  // rows in the current level's layout
-    const uint32_t* grow = nullptr;
+    uint32_t grow_v = 0;
     if (tk.cone) {
         rows = (levels > 0) ? s_bottom : (tk.hi - tk.lo);
         if (rows > cap_rows) {
@@ -774,33 +788,39 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
             __syncthreads();
             return;
         }
-        // global source row of every bottom row (in the table slot of the
-        // deepest level, which no merge level uses), then every level's map
-        uint32_t* gr = tab + (size_t)(kFfaMaxLevels - 1) * cap_rows;
-        for (int p = tid; p < rows; p += NT) {
+        // lane i holds the global source row of this wavefront's i-th LDS
+        // row (wave + i*NW), the row its loads are issued from
+        const int p = wave + lane * NW;
+        if (p < rows) {
             const Seg g = seg[seg_find(seg, levels, p)];
-            gr[p] = (uint32_t)(g.r0 + g.lo + (p - g.base));
+            grow_v = (uint32_t)(g.r0 + g.lo + (p - g.base));
         }
-        // one flat loop over (level, row) so that every thread has work
-        for (int e = tid; e < levels * cap_rows; e += NT) {
-            const int t = e / cap_rows;
-            const int p = e - t * cap_rows;
-            if (p < level_rows(true, seg, t, 0)) {
-                int a, b, sh;
-                cone_row_map(seg, t, p, bins, M, &s_coef, &a, &b, &sh);
-                tab[e] = pack_row(a, b, sh);
-            }
-        }
-        grow = gr;
     } else {
         rows = M;
-        for (int p = tid; p < rows; p += NT)
-            identity_row_tables(M, levels, p, bins, &s_coef, tab, cap_rows);
     }
-    __syncthreads();
+    // the row tables of every level, built while the row loads are in flight
+    const int rows0 = rows;
+    auto tables = [&, rows0]() {
+        if (tk.cone) {
+            // one flat loop over (level, row) so that every thread has work
+            for (int e = tid; e < levels * cap_rows; e += NT) {
+                const int t = e / cap_rows;
+                const int p = e - t * cap_rows;
+                if (p < level_rows(true, seg, t, 0)) {
+                    int a, b, sh;
+                    cone_row_map(seg, t, p, bins, M, &s_coef, &a, &b, &sh);
+                    tab[e] = pack_row(a, b, sh);
+                }
+            }
+        } else {
+            for (int p = tid; p < rows0; p += NT)
+                identity_row_tables(M, levels, p, bins, &s_coef, tab, cap_rows);
+        }
+    };
 
-    // ---- staging, merge levels (and the store of kind-0 tasks)
-    rows = run_rows<NW, STAGE, V>(tk, seg, data, tab, grow, rows, pitch, wave, lane);
+    // ---- staging (tables meanwhile), merge levels (and the store of kind-0
+    // tasks)
+    rows = run_rows<NW, STAGE, V>(tk, seg, data, tab, grow_v, rows, pitch, wave, lane, tables);
     if (rows < 0) {
         // a row length this kernel variant was not built for
         if (tid == 0) atomicOr(err, 2);

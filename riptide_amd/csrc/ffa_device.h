@@ -48,25 +48,25 @@ __device__ __forceinline__ void ff_add(float ah, float al, float bh, float bl, f
     rl = e - (rh - s);
 }
 
-// Maximum over the 64 lanes of a wavefront with DPP moves (no LDS traffic),
-// returned to every lane.
+// Inclusive prefix sum over the 64 lanes of a wavefront with DPP moves (no
+// LDS traffic): shifts by 1, 2, 4, 8 lanes within each 16-lane row, then the
+// row totals broadcast into the rows above.  A lane whose DPP source lies
+// outside the pattern (or in a row the mask leaves out) adds 0.
 template <int CTRL, int ROW_MASK>
-__device__ __forceinline__ float dpp_max(float v)
+__device__ __forceinline__ int dpp_add(int v)
 {
-    const int o = __builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), CTRL, ROW_MASK,
-                                              0xF, false);
-    return fmaxf(v, __int_as_float(o));
+    return v + __builtin_amdgcn_update_dpp(0, v, CTRL, ROW_MASK, 0xF, false);
 }
 
-__device__ __forceinline__ float wave_max_all(float v)
+__device__ __forceinline__ int wave_inclusive_sum(int v)
 {
-    v = dpp_max<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
-    v = dpp_max<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
-    v = dpp_max<0x141, 0xF>(v);  // row_half_mirror: 8-lane groups
-    v = dpp_max<0x140, 0xF>(v);  // row_mirror: 16-lane rows
-    v = dpp_max<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
-    v = dpp_max<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), kWave - 1));
+    v = dpp_add<0x111, 0xF>(v);  // row_shr:1
+    v = dpp_add<0x112, 0xF>(v);  // row_shr:2
+    v = dpp_add<0x114, 0xF>(v);  // row_shr:4
+    v = dpp_add<0x118, 0xF>(v);  // row_shr:8
+    v = dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    return v;
 }
 
 __device__ __forceinline__ float wave_max(float v)

@@ -130,18 +130,14 @@ __device__ __forceinline__ void build_cone_wave(Seg* seg, const FfaTask& tk, con
         }
         // exclusive scan of the children's lengths, children 2j and 2j+1 on lane j
         const int run = a.len + b.len;
-        int incl = run;
-        for (int d = 1; d < kWave; d <<= 1) {
-            const int o = __shfl_up(incl, d, kWave);
-            if (lane >= d) incl += o;
-        }
+        const int incl = wave_inclusive_sum(run);
         a.base = incl - run;
         b.base = a.base + a.len;
         if (j < (1 << t)) {
             seg[seg_slot(t + 1, 2 * j)] = a;
             seg[seg_slot(t + 1, 2 * j + 1)] = b;
         }
-        bottom = __shfl(incl, kWave - 1, kWave);
+        bottom = __builtin_amdgcn_readlane(incl, kWave - 1);
     }
     if (lane == 0) *bottom_rows = bottom;
     __builtin_amdgcn_wave_barrier();
@@ -477,11 +473,14 @@ __device__ __forceinline__ void window_max_chunk(const float* cps, int i0, int i
                     for (int u = 0; u < 4; ++u) q[jj][u] = cps[i + u + w[j0 + jj]];
                 }
             asm volatile("" ::: "memory");
+            // two three-operand maxima per four windows
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj)
-                if (j0 + jj < NWC)
-                    m[j0 + jj] = fmaxf(m[j0 + jj], fmaxf(fmaxf(q[jj][0] - c[0], q[jj][1] - c[1]),
-                                                         fmaxf(q[jj][2] - c[2], q[jj][3] - c[3])));
+                if (j0 + jj < NWC) {
+                    const int j = j0 + jj;
+                    m[j] = fmaxf(fmaxf(m[j], q[jj][0] - c[0]), q[jj][1] - c[1]);
+                    m[j] = fmaxf(fmaxf(m[j], q[jj][2] - c[2]), q[jj][3] - c[3]);
+                }
         }
     }
     for (; i < plain_end; ++i) {

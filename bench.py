@@ -33,11 +33,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def make_inputs(torch, n_series, nsamp, tsamp, seed, dev):
-    """White + red noise (PSD ~ f^-1.5, equal to the white level at 0.1 Hz),
-    one series in 16 with a Gaussian pulse train (P log-uniform 0.2-8 s, duty
-    1-5 %, S/N 8-30); generated on the GPU.  nsamp: one length for all series
-    or a sequence of per-series lengths."""
+def make_inputs(torch, n_series, nsamp, tsamp, seed, dev, red_index=-1.5):
+    """White + red noise (PSD ~ f^red_index, equal to the white level at
+    0.1 Hz), one series in 16 with a Gaussian pulse train (P log-uniform 0.2-8
+    s, duty 1-5 %, S/N 8-30); generated on the GPU.  nsamp: one length for all
+    series or a sequence of per-series lengths."""
     g = torch.Generator(device=dev)
     g.manual_seed(seed)
     rs = np.random.RandomState(seed)
@@ -47,7 +47,7 @@ def make_inputs(torch, n_series, nsamp, tsamp, seed, dev):
     nfft = 1 << int(np.ceil(np.log2(max(lengths))))
     f = torch.fft.rfftfreq(nfft, d=tsamp, device=dev)
     shape = torch.zeros_like(f)
-    shape[1:] = (f[1:] / 0.1) ** (-0.75)
+    shape[1:] = (f[1:] / 0.1) ** (0.5 * red_index)
     xs = []
     for i in range(n_series):
         nsamp = lengths[i]
@@ -143,6 +143,88 @@ def fwd_bwd_timing(torch, _lib, reps=5):
             "fwd_ms": round(f, 3), "fwd_bwd_ms": round(fb, 3), "ratio": round(fb / f, 3)}
 
 
+def c5_timing(torch, _lib, seed, adam_steps=50, nseries=128, nsamp=2 ** 21, tsamp=128e-6,
+              ntaps=129, reps=2):
+    """Batched forward + adjoint with FIR whitening: nseries x nsamp at tsamp,
+    red noise (PSD index -2.0, equal to the white level at 0.1 Hz) plus one
+    Gaussian pulse train per series (P log-uniform 0.5-5 s, duty 2 %, S/N 15).
+    Per series: FIR (ntaps, identity at the start) -> normalise ->
+    periodogram over P 0.5-5 s, bins 240-260; objective = sum over widths of
+    (max S/N within +-0.5 % of the series' period) - (median S/N).  Times the
+    forward and forward + adjoint of the whole batch (HIP events, median of
+    `reps`), runs `adam_steps` Adam steps and compares the objective with
+    ffa_search's running-median dereddening (4 s)."""
+    from riptide_amd.autograd import periodogram_fn, periodogram_grid
+    from riptide_amd.ffautils import generate_width_trials
+    from riptide_amd.fir import fir_filter, identity_taps, near_period_rows, optimise_whitening
+    from riptide_amd.fir import whitening_objective
+    from riptide_amd.search import ffa_search
+    from riptide_amd.time_series import TimeSeries
+    dev = _lib.device()
+    rs = np.random.RandomState(seed + 5)
+    periods = np.exp(rs.uniform(np.log(0.5), np.log(5.0), size=nseries))
+    xs = make_inputs(torch, nseries, nsamp, tsamp, seed + 5, dev, red_index=-2.0)
+    t = torch.arange(nsamp, device=dev, dtype=torch.float64) * tsamp
+    for i in range(nseries):
+        ph = torch.remainder(t / float(periods[i]), 1.0)
+        sig = torch.exp(-0.5 * ((ph - 0.5) / (0.02 / 2.355)) ** 2)
+        sig *= 15.0 / torch.sqrt((sig ** 2).sum())
+        xs[i] += sig.float()
+    X = torch.stack(xs)
+    del xs
+    kw = dict(period_min=0.5, period_max=5.0, bins_min=240, bins_max=260)
+    widths = generate_width_trials(240)
+    grid, _ = periodogram_grid(nsamp, tsamp, 0.5, 5.0, 240, 260)
+    rows = [near_period_rows(grid, p) for p in periods]
+
+    def run(backward):
+        taps = identity_taps(nseries, ntaps).requires_grad_(backward)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        y = fir_filter(X, taps)
+        total = 0.0
+        for s in range(nseries):
+            total = total + whitening_objective(
+                periodogram_fn(y[s], tsamp, widths, **kw), rows[s])
+        if backward:
+            (-total).backward()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    with torch.no_grad():
+        run(False)
+    f = sorted(run(False) for _ in range(reps))[reps // 2]
+    fb = sorted(run(True) for _ in range(reps))[reps // 2]
+    log(f"c5: fwd {f:.1f} ms, fwd+bwd {fb:.1f} ms per batch of {nseries}")
+    t0 = time.time()
+    _, hist = optimise_whitening(X, tsamp, periods, steps=adam_steps, ntaps=ntaps, lr=2e-3,
+                                 log=log, **kw)
+    torch.cuda.synchronize()
+    t_adam = time.time() - t0
+    log(f"c5: {adam_steps} Adam steps in {t_adam:.1f} s, mean objective "
+        f"{hist[0].mean():.3f} -> {hist[-1].mean():.3f}")
+    # the fallback and judge: running-median dereddening (ffa_search)
+    rm = []
+    for s in range(nseries):
+        ts = TimeSeries.from_numpy_array(_lib.to_numpy(X[s]), tsamp)
+        _, pg = ffa_search(ts, rmed_width=4.0, **kw)
+        S = np.asarray(pg.snrs)
+        rm.append(float((S[rows[s]].max(axis=0) - np.median(S, axis=0)).sum()))
+    rm = np.asarray(rm)
+    return {"config": (f"{nseries} series x 2^{int(np.log2(nsamp))} samples, tsamp "
+                       f"{tsamp * 1e6:.0f} us, red noise index -2.0 + one pulse train per series "
+                       f"(P log-uniform 0.5-5 s, duty 2 %, S/N 15); FIR {ntaps} taps -> "
+                       f"normalise -> periodogram P 0.5-5 s, bins 240-260"),
+            "fwd_ms": round(f, 2), "fwd_bwd_ms": round(fb, 2), "ratio": round(fb / f, 3),
+            "adam_steps": adam_steps, "adam_s": round(t_adam, 2),
+            "objective_identity": round(float(hist[0].mean()), 3),
+            "objective_optimised": round(float(hist[-1].mean()), 3),
+            "objective_running_median": round(float(rm.mean()), 3),
+            "optimised_beats_running_median": int((hist[-1] > rm).sum())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -153,6 +235,8 @@ def main():
     ap.add_argument("--plain-trials", type=int, default=16)
     ap.add_argument("--group-size", type=int, default=64)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c5-steps", type=int, default=50,
+                    help="Adam steps of the FIR-whitening run (0: skip it)")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--seed", type=int, default=20260)
     ap.add_argument("--no-c3", action="store_true",
@@ -312,14 +396,18 @@ def main():
     # differentiable search (config 4 shape): forward vs forward + adjoint
     grad = fwd_bwd_timing(torch, _lib)
     log(f"fwd+bwd: {grad}")
+    c5 = c5_timing(torch, _lib, args.seed, adam_steps=args.c5_steps) if args.c5_steps > 0 else None
 
     # roofline of the dominant kernel: persistent FFA + S/N kernel
     snr_bytes = sum(batch.lengths[i] * batch.widths[i].size * 4 for i in range(args.trials))
     fold_bytes = st["fold_floats"] * 4
+    # the unit of the roofline figures is one group of series: its FFA
+    # launches, one per kernel variant (row-length class) present in it
+    groups = max(1, (args.trials + args.group_size - 1) // args.group_size)
     launches = max(1, phases["ffa_launches"])
-    bytes_per_launch = (fold_bytes + snr_bytes) / launches
-    ms_per_launch = phases["ffa_ms"] / launches
-    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
+    bytes_per_group = (fold_bytes + snr_bytes) / groups
+    ms_per_group = phases["ffa_ms"] / groups
+    achieved = bytes_per_group / (ms_per_group * 1e-3) / 1e9
     lds_bytes = 8 * st["ffa_adds"]
     lds_achieved = lds_bytes / (phases["ffa_ms"] * 1e-3) / 1e9
 
@@ -332,7 +420,8 @@ def main():
     if (os.path.exists(tpath) and args.trials == 512 and args.nsamp == 2 ** 22
             and args.group_size == 64):
         with open(tpath) as f:
-            traffic = int(json.load(f)["hbm_bytes_per_launch"])
+            tj = json.load(f)
+        traffic = int(tj["hbm_bytes_per_group"]) if "hbm_bytes_per_group" in tj else None
 
     cpu = None
     if not args.no_cpu:
@@ -370,7 +459,8 @@ def main():
         "parity": parity,
         "roofline": {
             "bound": "hbm",
-            "kernel": "ffa_batch_kernel (fold + FFA merge levels + boxcar S/N)",
+            "kernel": ("ffa_batch_kernel (fold + FFA merge levels + boxcar S/N); figures per "
+                       "group of series: its launches, one per kernel variant"),
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -378,14 +468,16 @@ def main():
             "traffic": traffic,
             "traffic_source": ("profiles/ffa_batch_kernel__batch512x2p22_traffic.json"
                                if traffic is not None else None),
-            "algorithmic_bytes_per_launch": int(bytes_per_launch),
-            "ms_per_launch": round(ms_per_launch, 3),
+            "algorithmic_bytes_per_group": int(bytes_per_group),
+            "ms_per_group": round(ms_per_group, 3),
+            "launches_per_group": round(launches / groups, 3),
             "lds": {"achieved": round(lds_achieved, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
                     "frac": round(lds_achieved / LDS_PEAK_GBS, 4),
                     "bytes": "8 B per FFA float add"},
         },
         "heterogeneous_batch": c3,
         "fwd_bwd": grad,
+        "fir_whitening_batch": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -190,7 +190,7 @@ int rt_batch_run(void* handle, int use_graph, void* stream);
 
 /* One eager run with HIP events around each phase of each group, on
  * `stream`; waits.  ms[0..3] = preparation ms, downsampling ms, FFA + S/N ms,
- * number of FFA launches. */
+ * number of FFA launches (one per group and FFA kernel variant). */
 int rt_batch_run_timed(void* handle, void* stream, double* ms);
 
 /* Waits for `stream` and reports a device-side tiling error, if any. */
@@ -201,6 +201,32 @@ int rt_batch_check(void* handle, void* stream);
 int rt_batch_stats(void* handle, int64_t* stats);
 
 int rt_batch_destroy(void* handle);
+
+/* ---- FIR whitening front end of a differentiable batched search (no
+ * counterpart in the reference, which dereddens with a running median only,
+ * time_series.py:95-124; the filters are optimised for S/N by gradient
+ * descent through rt_periodogram_backward) ----------------------------------- */
+
+/* nseries series of size samples each (row-major, d_x), each with its own odd
+ * number ntaps of taps (row-major, d_taps), centre c = (ntaps - 1)/2, zero
+ * outside the series: y[s][n] = sum_k taps[s][k] * x[s][n + k - c].  With
+ * reverse the taps are applied in reverse order, which gives the gradient
+ * with respect to x of sum(g * y) from g. */
+int rt_fir_forward(const float* d_x, int64_t nseries, int64_t size, const float* d_taps,
+                   int64_t ntaps, int reverse, float* d_y, void* stream);
+
+/* Floats of device workspace rt_fir_grad_taps needs. */
+int rt_fir_workspace_floats(int64_t nseries, int64_t size, int64_t ntaps, int64_t* floats);
+
+/* Gradient of sum(g * y) with respect to the taps:
+ * gtaps[s][k] = sum_n g[s][n] * x[s][n + k - c] (nseries x ntaps floats). */
+int rt_fir_grad_taps(const float* d_x, const float* d_g, int64_t nseries, int64_t size,
+                     int64_t ntaps, float* d_work, float* d_gtaps, void* stream);
+
+/* One Adam step (Kingma & Ba 2015) on count parameters, step counted from 1;
+ * d_m and d_v hold the moment estimates (zero before the first step). */
+int rt_adam_update(float* d_params, const float* d_grad, float* d_m, float* d_v, int64_t count,
+                   double lr, double beta1, double beta2, double eps, int64_t step, void* stream);
 
 #ifdef __cplusplus
 }

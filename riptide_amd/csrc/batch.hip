@@ -550,7 +550,8 @@ int rt_batch_run(void* handle, int use_graph, void* stream)
 
 int rt_batch_run_timed(void* handle, void* stream, double* ms)
 {
-    // ms[0] preparation, ms[1] downsampling, ms[2] FFA + S/N, ms[3] FFA launches
+    // ms[0] preparation, ms[1] downsampling, ms[2] FFA + S/N, ms[3] FFA launches (one per
+    // kernel variant per group)
     Batch* b = (Batch*)handle;
     if (!b->bound) return bfail(RT_EINVAL, "rt_batch_bind must be called before rt_batch_run");
     hipStream_t s = (hipStream_t)stream;
@@ -570,7 +571,9 @@ int rt_batch_run_timed(void* handle, void* stream, double* ms)
         ms[0] = acc[0];
         ms[1] = acc[1];
         ms[2] = acc[2];
-        ms[3] = (double)b->groups.size();
+        size_t runs = 0;
+        for (const Group& g : b->groups) runs += g.runs.size();
+        ms[3] = (double)runs;
     }
     for (auto& e : pt.ev) hipEventDestroy(e);
     return st;

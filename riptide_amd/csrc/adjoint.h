@@ -18,9 +18,15 @@ hipError_t launch_snr_backward(const float* x, int rows, int bins, const float* 
 hipError_t launch_ffa_adjoint_level(const float* gp, float* gc, int64_t rows, int64_t cols,
                                     int depth, hipStream_t s);
 
-// Transpose of downsampling by f: gin[N] (+)= D^T gout[n].
+// Transpose of downsampling by f: gin[N] (+)= D^T gout[n]; idx is n int4 of
+// workspace (the index table of the downsampling).
 hipError_t launch_downsample_adjoint(const float* gout, int64_t n, double f, int64_t N,
-                                     float* gin, int accumulate, hipStream_t s);
+                                     float* gin, int accumulate, int4* idx, hipStream_t s);
+
+// flags[t] = any non-zero S/N gradient in rows [rows[t].x, rows[t].x + rows[t].y) of gsnr
+// (nw per row).
+hipError_t launch_transform_need(const float* gsnr, const int2* rows, int ntr, int nw, int* flags,
+                                 hipStream_t s);
 
 // gin[i] (+)= gout[i] for i < n, 0 beyond.
 hipError_t launch_add_prefix(const float* gout, int64_t n, int64_t N, float* gin, int accumulate,

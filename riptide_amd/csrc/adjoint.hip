@@ -165,8 +165,26 @@ __global__ void ffa_adjoint_level_kernel(const float* __restrict__ gp, float* __
 // collects the weights it had in the (at most three) outputs that read it.
 // ---------------------------------------------------------------------------
 
+// Index table of the downsampling by f: output k reads inputs [imin, imax]
+// with edge weights wmin, wmax (the exact float64 index arithmetic of the
+// reference, evaluated once per output).
+__global__ void downsample_index_kernel(int64_t n, uint64_t mf, int ef, int64_t N,
+                                        int4* __restrict__ idx)
+{
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+         k += (int64_t)gridDim.x * blockDim.x) {
+        const DsIndex ix = ds_index(k, mf, ef, N);
+        idx[k] = make_int4((int)ix.imin, (int)ix.imax, __float_as_int(ix.wmin),
+                           __float_as_int(ix.wmax));
+    }
+}
+
+// Every input sample i gathers the weights it had in the outputs that read
+// it: those lie among k = floor(i/f) - 2 .. floor(i/f) + 3 (an output spans
+// at least one input, its ends are floor(k*f) and floor(k*f + f) to within
+// one sample, and i*(1/f) in float32 is within one of i/f).
 __global__ void downsample_adjoint_kernel(const float* __restrict__ gout, int64_t n,
-                                          uint64_t mf, int ef, float inv_f, int64_t N,
+                                          const int4* __restrict__ idx, float inv_f, int64_t N,
                                           float* __restrict__ gin, int accumulate)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
@@ -175,12 +193,12 @@ __global__ void downsample_adjoint_kernel(const float* __restrict__ gout, int64_
         if (k0 < 0) k0 = 0;
         float acc = 0.0f;
         for (int64_t k = k0; k < k0 + 6 && k < n; ++k) {
-            const DsIndex ix = ds_index(k, mf, ef, N);
-            if (i < ix.imin || i > ix.imax) continue;
+            const int4 ix = idx[k];
+            if (i < ix.x || i > ix.y) continue;
             float w = 0.0f;
-            if (i == ix.imin) w += ix.wmin;
-            if (i == ix.imax) w += ix.wmax;
-            if (i > ix.imin && i < ix.imax) w += 1.0f;
+            if (i == ix.x) w += __int_as_float(ix.z);
+            if (i == ix.y) w += __int_as_float(ix.w);
+            if (i > ix.x && i < ix.y) w += 1.0f;
             acc += w * gout[k];
         }
         gin[i] = accumulate ? gin[i] + acc : acc;
@@ -237,17 +255,34 @@ __global__ __launch_bounds__(256) void normalise_adjoint_partial_kernel(
     }
 }
 
-__global__ void normalise_adjoint_apply_kernel(const float* __restrict__ g,
-                                               const float* __restrict__ y, int64_t N,
-                                               const float* __restrict__ part, int nparts,
-                                               float inv_std, float* __restrict__ gx)
+// The partial float-float sums are reduced by every block in parallel (lane
+// p of the block takes partials p, p + blockDim, ...), then applied.
+__global__ __launch_bounds__(256) void normalise_adjoint_apply_kernel(
+    const float* __restrict__ g, const float* __restrict__ y, int64_t N,
+    const float* __restrict__ part, int nparts, float inv_std, float* __restrict__ gx)
 {
+    __shared__ float sh[4][4];
     __shared__ float s_mg, s_mgy;
+    float a = 0.0f, al = 0.0f, b = 0.0f, bl = 0.0f;
+    for (int p = threadIdx.x; p < nparts; p += blockDim.x) {
+        ff_add(a, al, part[4 * p], part[4 * p + 1], a, al);
+        ff_add(b, bl, part[4 * p + 2], part[4 * p + 3], b, bl);
+    }
+    for (int d = 32; d >= 1; d >>= 1) {
+        const float oa = __shfl_down(a, d, kWave), oal = __shfl_down(al, d, kWave);
+        const float ob = __shfl_down(b, d, kWave), obl = __shfl_down(bl, d, kWave);
+        ff_add(a, al, oa, oal, a, al);
+        ff_add(b, bl, ob, obl, b, bl);
+    }
+    const int lane = threadIdx.x & 63, wave = threadIdx.x / 64;
+    if (lane == 0) {
+        sh[wave][0] = a; sh[wave][1] = al; sh[wave][2] = b; sh[wave][3] = bl;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
-        float a = 0.0f, al = 0.0f, b = 0.0f, bl = 0.0f;
-        for (int p = 0; p < nparts; ++p) {
-            ff_add(a, al, part[4 * p], part[4 * p + 1], a, al);
-            ff_add(b, bl, part[4 * p + 2], part[4 * p + 3], b, bl);
+        for (int w = 1; w < (int)(blockDim.x / 64); ++w) {
+            ff_add(a, al, sh[w][0], sh[w][1], a, al);
+            ff_add(b, bl, sh[w][2], sh[w][3], b, bl);
         }
         s_mg = (a + al) / (float)N;
         s_mgy = (b + bl) / (float)N;
@@ -257,6 +292,21 @@ __global__ void normalise_adjoint_apply_kernel(const float* __restrict__ g,
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * blockDim.x)
         gx[i] = (g[i] - mg - y[i] * mgy) * inv_std;
+}
+
+// flags[t] = 1 if any of the rows_eval x nw S/N gradients of transform t
+// (starting at row out_row) is non-zero; one block per transform.
+__global__ __launch_bounds__(256) void transform_need_kernel(const float* __restrict__ gsnr,
+                                                             const int2* __restrict__ rows,
+                                                             int nw, int* __restrict__ flags)
+{
+    const int2 r = rows[blockIdx.x];
+    const float* g = gsnr + (size_t)r.x * nw;
+    const int64_t cnt = (int64_t)r.y * nw;
+    int any = 0;
+    for (int64_t e = threadIdx.x; e < cnt; e += blockDim.x) any |= g[e] != 0.0f;
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) flags[blockIdx.x] = any;
 }
 
 // ---------------------------------------------------------------------------
@@ -293,13 +343,15 @@ hipError_t launch_ffa_adjoint_level(const float* gp, float* gc, int64_t rows, in
 }
 
 hipError_t launch_downsample_adjoint(const float* gout, int64_t n, double f, int64_t N,
-                                     float* gin, int accumulate, hipStream_t s)
+                                     float* gin, int accumulate, int4* idx, hipStream_t s)
 {
     uint64_t mf;
     int ef;
     split_double(f, &mf, &ef);
+    hipLaunchKernelGGL(downsample_index_kernel, dim3(grid_of(n, 256)), dim3(256), 0, s, n, mf, ef,
+                       N, idx);
     hipLaunchKernelGGL(downsample_adjoint_kernel, dim3(grid_of(N, 256)), dim3(256), 0, s, gout, n,
-                       mf, ef, (float)(1.0 / f), N, gin, accumulate);
+                       (const int4*)idx, (float)(1.0 / f), N, gin, accumulate);
     return hipGetLastError();
 }
 
@@ -318,6 +370,14 @@ hipError_t launch_normalise_adjoint(const float* g, const float* y, int64_t N, f
                        part);
     hipLaunchKernelGGL(normalise_adjoint_apply_kernel, dim3(grid_of(N, 256)), dim3(256), 0, s, g,
                        y, N, (const float*)part, nparts, inv_std, gx);
+    return hipGetLastError();
+}
+
+hipError_t launch_transform_need(const float* gsnr, const int2* rows, int ntr, int nw, int* flags,
+                                 hipStream_t s)
+{
+    if (ntr <= 0) return hipSuccess;
+    hipLaunchKernelGGL(transform_need_kernel, dim3(ntr), dim3(256), 0, s, gsnr, rows, nw, flags);
     return hipGetLastError();
 }
 

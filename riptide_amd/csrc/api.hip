@@ -894,21 +894,30 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
         RT_HIP(hipStreamSynchronize(s));
         return RT_OK;
     }
-    // which transforms receive any gradient: per-row flags from the host copy
-    std::vector<float> g((size_t)(p.length * nw));
-    RT_HIP(hipMemcpyAsync(g.data(), d_gsnr, g.size() * sizeof(float), hipMemcpyDeviceToHost, s));
+    // which transforms receive any gradient: flags computed on the device
+    // from each transform's rows of d_gsnr (only the flags cross to the host)
+    const int ntr = (int)p.tr.size();
+    std::vector<int32_t> meta((size_t)3 * ntr);
+    for (int it = 0; it < ntr; ++it) {
+        meta[2 * it] = (int32_t)p.tr[it].out_row;
+        meta[2 * it + 1] = (int32_t)p.tr[it].rows_eval;
+    }
+    int32_t* d_meta = nullptr;
+    st = upload(kSlotMisc, meta, &d_meta, s);
+    if (st) return st;
+    int32_t* d_flags = d_meta + 2 * ntr;
+    RT_HIP(launch_transform_need(d_gsnr, reinterpret_cast<const int2*>(d_meta), ntr, (int)nw,
+                                 d_flags, s));
+    std::vector<int32_t> flags(ntr);
+    RT_HIP(hipMemcpyAsync(flags.data(), d_flags, (size_t)ntr * sizeof(int32_t),
+                          hipMemcpyDeviceToHost, s));
     RT_HIP(hipStreamSynchronize(s));
     std::vector<char> need_tr(p.tr.size(), 0), need_step(p.steps.size(), 0);
-    for (size_t it = 0; it < p.tr.size(); ++it) {
-        const Transform& t = p.tr[it];
-        const float* gt = g.data() + (size_t)t.out_row * nw;
-        for (int64_t q = 0; q < (int64_t)t.rows_eval * nw; ++q)
-            if (gt[q] != 0.0f) {
-                need_tr[it] = 1;
-                need_step[t.step] = 1;
-                break;
-            }
-    }
+    for (int it = 0; it < ntr; ++it)
+        if (flags[it]) {
+            need_tr[it] = 1;
+            need_step[p.tr[it].step] = 1;
+        }
     // S/N tables for bins in [bmin, bmax]
     std::vector<float> tab = hb_table(bmin, bmax - bmin + 1, h_widths, nw);
     const size_t woff = tab.size();
@@ -920,14 +929,17 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
     float* d_tab = nullptr;
     st = upload(kSlotTables, tab, &d_tab, s);
     if (st) return st;
-    // workspace: ds values + ds gradient (max n) and three transform buffers
+    // workspace: ds values + ds gradient + downsampling index table (max n)
+    // and three transform buffers
     int64_t max_fold = 1;
     for (size_t it = 0; it < p.tr.size(); ++it)
         if (need_tr[it]) max_fold = std::max<int64_t>(max_fold, (int64_t)p.tr[it].rows * p.tr[it].bins);
     void* wsp = nullptr;
-    st = pool_get(kSlotInter, (size_t)(2 * p.max_n + 4 * max_fold) * sizeof(float), &wsp);
+    st = pool_get(kSlotInter, (size_t)(6 * p.max_n + 4 * max_fold) * sizeof(float), &wsp);
     if (st) return st;
-    float* ds = (float*)wsp;
+    // the index table first: int4 entries need 16-byte alignment
+    int4* dsidx = reinterpret_cast<int4*>(wsp);
+    float* ds = reinterpret_cast<float*>(dsidx + p.max_n);
     float* gds = ds + p.max_n;
     float* T = gds + p.max_n;
     float* A = T + max_fold;
@@ -938,24 +950,21 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
     if (st) return st;
     RT_HIP(hipMemsetAsync(errp, 0, sizeof(int), s));
 
+    // the forward tasks of every transform to recompute, uploaded at once
+    const int shape = current_ffa_shape();
+    std::vector<FfaTask> all;
+    std::vector<std::vector<size_t>> pass_ends(p.tr.size());
+    std::vector<size_t> tr_first(p.tr.size(), 0);
     for (size_t is = 0; is < p.steps.size(); ++is) {
         if (!need_step[is]) continue;
         const Step& stp = p.steps[is];
-        const float* x = d_in;
-        if (stp.ds_offset >= 0) {
-            RT_HIP(launch_downsample_direct(d_in, size, stp.f, stp.n, ds, s));
-            x = ds;
-        }
-        RT_HIP(launch_fill_zero(gds, stp.n, s));
+        const float* x = stp.ds_offset >= 0 ? ds : d_in;
         for (int it = stp.first_tr; it < stp.first_tr + stp.ntr; ++it) {
             if (!need_tr[it]) continue;
             const Transform& t = p.tr[it];
-            const int64_t n = (int64_t)t.rows * t.bins;
-            // forward transform rows (recomputed)
             TransformSpec sp;
             sp.m = t.rows;
             sp.bins = t.bins;
-            const int shape = current_ffa_shape();
             sp.cap = ffa_cap_rows(t.bins, shape);
             sp.shape = shape;
             sp.src = x;
@@ -966,16 +975,32 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
             std::vector<std::vector<FfaTask>> passes;
             if (plan_transform_tasks(sp, passes) < 0)
                 return fail(RT_EINVAL, "bins too large for the FFA tiling");
-            std::vector<FfaTask> all;
-            for (auto& q : passes) all.insert(all.end(), q.begin(), q.end());
-            FfaTask* d_tasks = nullptr;
-            st = upload(kSlotTasks, all, &d_tasks, s);
-            if (st) return st;
-            size_t off = 0;
+            tr_first[it] = all.size();
             for (auto& q : passes) {
-                RT_HIP(launch_ffa_task_list(d_tasks + off, all.data() + off, (int)q.size(),
+                all.insert(all.end(), q.begin(), q.end());
+                pass_ends[it].push_back(all.size());
+            }
+        }
+    }
+    FfaTask* d_tasks = nullptr;
+    st = upload(kSlotTasks, all, &d_tasks, s);
+    if (st) return st;
+
+    for (size_t is = 0; is < p.steps.size(); ++is) {
+        if (!need_step[is]) continue;
+        const Step& stp = p.steps[is];
+        if (stp.ds_offset >= 0) RT_HIP(launch_downsample_direct(d_in, size, stp.f, stp.n, ds, s));
+        RT_HIP(launch_fill_zero(gds, stp.n, s));
+        for (int it = stp.first_tr; it < stp.first_tr + stp.ntr; ++it) {
+            if (!need_tr[it]) continue;
+            const Transform& t = p.tr[it];
+            const int64_t n = (int64_t)t.rows * t.bins;
+            // forward transform rows (recomputed)
+            size_t off = tr_first[it];
+            for (size_t e : pass_ends[it]) {
+                RT_HIP(launch_ffa_task_list(d_tasks + off, all.data() + off, (int)(e - off),
                                             (int*)errp, s, shape));
-                off += q.size();
+                off = e;
             }
             // dL/d(transform rows): S/N backward on the evaluated rows, zero beyond
             RT_HIP(launch_fill_zero(A, n, s));
@@ -994,11 +1019,9 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
             }
             // fold transpose: the first rows*bins samples, accumulated
             RT_HIP(launch_add_prefix(src, n, stp.n, gds, 1, s));
-            // the task list upload of the next transform reuses the slot
-            RT_HIP(hipStreamSynchronize(s));
         }
         if (stp.ds_offset >= 0)
-            RT_HIP(launch_downsample_adjoint(gds, stp.n, stp.f, size, d_gin, 1, s));
+            RT_HIP(launch_downsample_adjoint(gds, stp.n, stp.f, size, d_gin, 1, dsidx, s));
         else
             RT_HIP(launch_add_prefix(gds, stp.n, size, d_gin, 1, s));
     }

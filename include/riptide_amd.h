@@ -228,6 +228,29 @@ int rt_fir_grad_taps(const float* d_x, const float* d_g, int64_t nseries, int64_
 int rt_adam_update(float* d_params, const float* d_grad, float* d_m, float* d_v, int64_t count,
                    double lr, double beta1, double beta2, double eps, int64_t step, void* stream);
 
+/* ---- Peak detection on the device (peak_detection.py:38-240: find_peaks
+ * runs on host copies of the S/N array in the reference; here only the
+ * segment statistics and the candidate indices leave the device) ---------- */
+
+/* d_snrs: length x nw S/N (row-major).  For each of nseg consecutive segments
+ * of seglen rows and each width: the values of the given 0-based ranks
+ * (ascending order) within the segment, d_out[(seg * nw + w) * nranks + t].
+ * Exact order statistics (radix select); at most 8 ranks. */
+int rt_segment_select(const float* d_snrs, int64_t nw, int64_t nseg, int64_t seglen,
+                      const int32_t* h_ranks, int64_t nranks, float* d_out, void* stream);
+
+/* Peak candidates (peak_detection.py:154-158): rows ip where snr[ip][w] > smin
+ * (smin_lo: the largest float32 <= smin) and snr[ip][w] > poly_w(log f[ip]),
+ * d_coef holding ncoef float32 coefficients per width (highest degree first),
+ * d_logf float32 log-frequencies.  Sure passes are appended to d_sel[w * cap
+ * + ...], values within the float32 evaluation's error bound of the
+ * threshold to d_unc[w * cap + ...] (the caller decides those in float64);
+ * d_counts[2w] and d_counts[2w + 1] count them (must be zero on entry; may
+ * exceed cap, in which case the lists are incomplete). */
+int rt_peak_candidates(const float* d_snrs, int64_t length, int64_t nw, const float* d_logf,
+                       const float* d_coef, int64_t ncoef, float smin_lo, int32_t* d_counts,
+                       int32_t* d_sel, int32_t* d_unc, int64_t cap, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -225,6 +225,43 @@ def c5_timing(torch, _lib, seed, adam_steps=50, nseries=128, nsamp=2 ** 21, tsam
             "optimised_beats_running_median": int((hist[-1] > rm).sum())}
 
 
+def peak_timing(torch, batch, outs, bmin, tsamp, nsamp, ntrials=16, ncpu=2):
+    """find_peaks on the first `ntrials` S/N arrays of the batch with the
+    device path (PeakFinder per plan), and on `ncpu` of them with the host
+    path (device-to-host copy + the numpy restatement of the reference)."""
+    from oracle import peaks as OP
+    from riptide_amd.peak_detection import PeakFinder
+    tobs = nsamp * tsamp
+    finders = {}
+    t0 = time.perf_counter()
+    npk = 0
+    for i in range(ntrials):
+        P, F = batch.grid(i)
+        key = int(bmin[i])
+        if key not in finders:
+            finders[key] = PeakFinder(P, F, batch.widths[i], tobs)
+    t_plan = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(ntrials):
+        pk, _ = finders[int(bmin[i])].find(outs[i])
+        npk += len(pk)
+    torch.cuda.synchronize()
+    dev_ms = (time.perf_counter() - t0) / ntrials * 1e3
+    t0 = time.perf_counter()
+    same = True
+    for i in range(ncpu):
+        P, F = batch.grid(i)
+        S = outs[i].cpu().numpy()
+        want, _ = OP.find_peaks(P, F, batch.widths[i], S, tobs)
+        got, _ = finders[int(bmin[i])].find(outs[i])
+        same = same and [tuple(p) for p in got] == want
+    host_ms = (time.perf_counter() - t0) / ncpu * 1e3 - dev_ms
+    return {"trials": ntrials, "device_ms_per_trial": round(dev_ms, 2),
+            "host_path_ms_per_trial": round(host_ms, 2), "plan_setup_ms": round(t_plan * 1e3, 1),
+            "peaks_per_trial": round(npk / ntrials, 1), "same_peaks_as_host_path": bool(same)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,6 +365,12 @@ def main():
             "snr_within_1e-5rel_1e-4abs": bool(np.allclose(S0, So, rtol=1e-5, atol=1e-4)),
         }
         log(f"parity: {parity}")
+
+        # peak detection on the batch's S/N, left on the device (the consumer
+        # of the periodograms): device path vs the host path of the
+        # reference (copy of the S/N + numpy find_peaks, here the oracle)
+        peaks_info = peak_timing(torch, batch, outs, bmin, tsamp, args.nsamp)
+        log(f"find_peaks: {peaks_info}")
 
         # plain schedule: one series at a time, per-step launches, no graph
         nplain = min(args.plain_trials, args.trials)
@@ -476,6 +519,7 @@ def main():
                     "bytes": "8 B per FFA float add"},
         },
         "heterogeneous_batch": c3,
+        "find_peaks": peaks_info,
         "fwd_bwd": grad,
         "fir_whitening_batch": c5,
         "cpu_baseline": cpu,

@@ -228,6 +228,20 @@ int rt_fir_grad_taps(const float* d_x, const float* d_g, int64_t nseries, int64_
 int rt_adam_update(float* d_params, const float* d_grad, float* d_m, float* d_v, int64_t count,
                    double lr, double beta1, double beta2, double eps, int64_t step, void* stream);
 
+/* ---- Candidate folding (folding.py:21-85; the reference folds on the host
+ * through libcpp.downsample) ------------------------------------------------ */
+
+/* The rows x bins block of a downsampled series (d_x, modified in place) is
+ * scaled by the float32 factor `scale`; with sum, d_out[b] receives the sum of
+ * the scaled rows (added row after row). */
+int rt_fold_rows(float* d_x, int64_t rows, int64_t bins, float scale, int sum, float* d_out,
+                 void* stream);
+
+/* downsample_vertical (folding.py:6-18): every column of a rows x bins array
+ * downsampled by f (1 < f < rows) into floor(rows / f) rows. */
+int rt_downsample_vertical(const float* d_x, int64_t rows, int64_t bins, double f, float* d_out,
+                           void* stream);
+
 /* ---- Peak detection on the device (peak_detection.py:38-240: find_peaks
  * runs on host copies of the S/N array in the reference; here only the
  * segment statistics and the candidate indices leave the device) ---------- */

@@ -65,6 +65,8 @@ _SIGNATURES = {
     "rt_fir_grad_taps": [_c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr, _c_ptr],
     "rt_adam_update": [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_dbl, _c_dbl, _c_dbl, _c_dbl,
                        _c_i64, _c_ptr],
+    "rt_fold_rows": [_c_ptr, _c_i64, _c_i64, _c_flt, ctypes.c_int, _c_ptr, _c_ptr],
+    "rt_downsample_vertical": [_c_ptr, _c_i64, _c_i64, _c_dbl, _c_ptr, _c_ptr],
     "rt_segment_select": [_c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_i64, _c_ptr, _c_ptr],
     "rt_peak_candidates": [_c_ptr, _c_i64, _c_i64, _c_ptr, _c_ptr, _c_i64, _c_flt, _c_ptr, _c_ptr,
                            _c_ptr, _c_i64, _c_ptr],

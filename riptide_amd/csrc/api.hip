@@ -420,6 +420,31 @@ int rt_downsample(const float* d_in, int64_t size, double f, float* d_out, void*
     return RT_OK;
 }
 
+int rt_fold_rows(float* d_x, int64_t rows, int64_t bins, float scale, int sum, float* d_out,
+                 void* stream)
+{
+    if (rows < 1 || bins < 1) return fail(RT_EINVAL, "nothing to fold");
+    hipStream_t s = S(stream);
+    RT_HIP(launch_fold_scale(d_x, rows * bins, scale, s));
+    if (sum) RT_HIP(launch_fold_sum_rows(d_x, rows, bins, d_out, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
+int rt_downsample_vertical(const float* d_x, int64_t rows, int64_t bins, double f, float* d_out,
+                           void* stream)
+{
+    // folding.py:9-14 (the checks of downsample_vertical) and downsample.hpp:11-16
+    if (!(f > 1.0)) return fail(RT_EINVAL, "factor must be > 1");
+    if (!(f < (double)rows))
+        return fail(RT_EINVAL, "factor must be strictly smaller than the number of input lines");
+    hipStream_t s = S(stream);
+    const int64_t n = downsampled_size(rows, f);
+    RT_HIP(launch_downsample_vertical(d_x, rows, bins, f, n, d_out, s));
+    RT_HIP(hipStreamSynchronize(s));
+    return RT_OK;
+}
+
 int rt_running_median(const float* d_in, int64_t size, int64_t width, float* d_out, void* stream)
 {
     // running_median.hpp:103-107

@@ -39,6 +39,47 @@ __global__ void downsample_direct_kernel(const float* __restrict__ in, int64_t N
     }
 }
 
+// Folding (folding.py:21-85): rows x bins samples scaled in place by a
+// float32 factor (`folded *= (m * factor) ** -0.5`).
+__global__ void fold_scale_kernel(float* __restrict__ x, int64_t count, float scale)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count;
+         i += (int64_t)gridDim.x * blockDim.x)
+        x[i] *= scale;
+}
+
+// Sum over the rows of a rows x bins array, row after row (numpy's reduction
+// over the outer axis of a C-ordered array adds the rows in order).
+__global__ void fold_sum_rows_kernel(const float* __restrict__ x, int64_t rows, int64_t bins,
+                                     float* __restrict__ out)
+{
+    for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < bins;
+         b += (int64_t)gridDim.x * blockDim.x) {
+        float acc = x[b];
+        for (int64_t i = 1; i < rows; ++i) acc += x[(size_t)i * bins + b];
+        out[b] = acc;
+    }
+}
+
+// Downsampling of every column of a rows x bins array by f
+// (folding.py:6-18): output row k of column b accumulates input rows
+// [imin, imax] of that column exactly as the one-series downsampling does.
+__global__ void downsample_vertical_kernel(const float* __restrict__ x, int64_t rows,
+                                           int64_t bins, uint64_t mf, int ef, int64_t n,
+                                           float* __restrict__ out)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n * bins;
+         e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = e / bins;
+        const int64_t b = e - k * bins;
+        const DsIndex ix = ds_index(k, mf, ef, rows);
+        const float* col = x + b;
+        float acc = ix.wmin * col[(size_t)ix.imin * bins];
+        for (int64_t i = ix.imin + 1; i < ix.imax; ++i) acc += col[(size_t)i * bins];
+        out[e] = __fmaf_rn(ix.wmax, col[(size_t)ix.imax * bins], acc);
+    }
+}
+
 // First output sample whose imin is >= c.
 __device__ __forceinline__ int64_t ds_first_at(int64_t c, const DsStepDev& st)
 {
@@ -628,6 +669,32 @@ hipError_t launch_downsample_direct(const float* in, int64_t N, double f, int64_
     split_double(f, &mf, &ef);
     hipLaunchKernelGGL(downsample_direct_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, in,
                        N, mf, ef, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_scale(float* x, int64_t count, float scale, hipStream_t stream)
+{
+    hipLaunchKernelGGL(fold_scale_kernel, dim3(grid_for(count, 256)), dim3(256), 0, stream, x,
+                       count, scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_sum_rows(const float* x, int64_t rows, int64_t bins, float* out,
+                                hipStream_t stream)
+{
+    hipLaunchKernelGGL(fold_sum_rows_kernel, dim3(grid_for(bins, 64)), dim3(64), 0, stream, x,
+                       rows, bins, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_downsample_vertical(const float* x, int64_t rows, int64_t bins, double f,
+                                      int64_t n, float* out, hipStream_t stream)
+{
+    uint64_t mf;
+    int ef;
+    split_double(f, &mf, &ef);
+    hipLaunchKernelGGL(downsample_vertical_kernel, dim3(grid_for(n * bins, 256)), dim3(256), 0,
+                       stream, x, rows, bins, mf, ef, n, out);
     return hipGetLastError();
 }
 

@@ -65,6 +65,12 @@ class TimeSeries(object):
             return TimeSeries(downsample(self.data, factor), factor * self.tsamp,
                               metadata=self.metadata)
 
+    def fold(self, period, bins, subints=None):
+        """Fold at `period` seconds into `bins` phase bins
+        (time_series.py:153-174, folding.py:21-85)."""
+        from .folding import fold
+        return fold(self, period, bins, subints=subints)
+
     @classmethod
     def generate(cls, length, tsamp, period, phi0=0.5, ducy=0.02, amplitude=10.0,
                  stdnoise=1.0):

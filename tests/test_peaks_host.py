@@ -65,3 +65,22 @@ def test_oracle_finds_planted_peaks():
                                   minseg=10 ** 6)
     assert polycos2[0] == [12.0]
     assert sorted({p[5] for p in pk2}) == sorted(at)
+
+
+def test_fold_argument_errors_match_reference():
+    """folding.py:52-67 checks, raised before any device work."""
+    import riptide_amd as rt
+    from riptide_amd.folding import downsample_vertical
+    ts = rt.TimeSeries.from_numpy_array(np.zeros(1000, np.float32), 1e-3)
+    with pytest.raises(ValueError, match="Period exceeds data length"):
+        ts.fold(2.0, 10)
+    with pytest.raises(ValueError, match="Bin width is shorter than sampling time"):
+        ts.fold(0.05, 100)
+    with pytest.raises(ValueError, match="subints must be >= 1 or None"):
+        ts.fold(0.1, 10, subints=0)
+    with pytest.raises(ValueError, match="exceeds the number of signal periods"):
+        ts.fold(0.3, 10, subints=4)
+    with pytest.raises(ValueError, match="factor must be > 1"):
+        downsample_vertical(np.zeros((5, 3)), 1.0)
+    with pytest.raises(ValueError, match="strictly smaller"):
+        downsample_vertical(np.zeros((5, 3)), 5.0)

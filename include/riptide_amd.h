@@ -228,6 +228,15 @@ int rt_fir_grad_taps(const float* d_x, const float* d_g, int64_t nseries, int64_
 int rt_adam_update(float* d_params, const float* d_grad, float* d_m, float* d_v, int64_t count,
                    double lr, double beta1, double beta2, double eps, int64_t step, void* stream);
 
+/* ---- Batched input staging (reading/presto.py, reading/sigproc.py and
+ * time_series.py:301-377 read one file at a time into host float32 arrays;
+ * here files are read into one pinned buffer, copied to the device in one
+ * transfer and 8-bit samples widened there) -------------------------------- */
+
+/* n 8-bit samples (unsigned, or signed when is_signed) at d_raw to float32. */
+int rt_convert_samples(const void* d_raw, int64_t n, int nbits, int is_signed, float* d_out,
+                       void* stream);
+
 /* ---- Candidate folding (folding.py:21-85; the reference folds on the host
  * through libcpp.downsample) ------------------------------------------------ */
 

@@ -65,6 +65,7 @@ _SIGNATURES = {
     "rt_fir_grad_taps": [_c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr, _c_ptr],
     "rt_adam_update": [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_i64, _c_dbl, _c_dbl, _c_dbl, _c_dbl,
                        _c_i64, _c_ptr],
+    "rt_convert_samples": [_c_ptr, _c_i64, ctypes.c_int, ctypes.c_int, _c_ptr, _c_ptr],
     "rt_fold_rows": [_c_ptr, _c_i64, _c_i64, _c_flt, ctypes.c_int, _c_ptr, _c_ptr],
     "rt_downsample_vertical": [_c_ptr, _c_i64, _c_i64, _c_dbl, _c_ptr, _c_ptr],
     "rt_segment_select": [_c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_i64, _c_ptr, _c_ptr],
@@ -133,6 +134,11 @@ def device():
 def stream():
     """Handle of the current HIP stream, as a C pointer."""
     return _c_ptr(torch().cuda.current_stream().cuda_stream)
+
+
+def stream_handle(s):
+    """Handle of a given torch stream, as a C pointer."""
+    return _c_ptr(s.cuda_stream)
 
 
 def ptr(t):

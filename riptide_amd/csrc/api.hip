@@ -420,6 +420,16 @@ int rt_downsample(const float* d_in, int64_t size, double f, float* d_out, void*
     return RT_OK;
 }
 
+int rt_convert_samples(const void* d_raw, int64_t n, int nbits, int is_signed, float* d_out,
+                       void* stream)
+{
+    // time_series.py:367-375: 8-bit samples are cast to float32
+    if (nbits != 8) return fail(RT_EINVAL, "only 8-bit samples need converting");
+    if (n < 1) return RT_OK;
+    RT_HIP(launch_convert_samples((const uint8_t*)d_raw, n, is_signed ? 2 : 1, d_out, S(stream)));
+    return RT_OK;
+}
+
 int rt_fold_rows(float* d_x, int64_t rows, int64_t bins, float scale, int sum, float* d_out,
                  void* stream)
 {

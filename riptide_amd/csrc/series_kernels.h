@@ -79,6 +79,9 @@ hipError_t launch_fused_rollback_add(const float* x, const float* y, int64_t siz
 hipError_t launch_circular_prefix_sum(const float* x, int64_t size, int64_t nsum, float* out,
                                       hipStream_t stream);
 hipError_t launch_fill_zero(float* p, int64_t n, hipStream_t stream);
+// 8-bit samples to float32 (kind 1: unsigned, 2: signed).
+hipError_t launch_convert_samples(const uint8_t* raw, int64_t n, int kind, float* out,
+                                  hipStream_t stream);
 // Folding: in-place float32 scaling, the sum of the rows of a rows x bins
 // array (rows added in order) and the downsampling of its columns.
 hipError_t launch_fold_scale(float* x, int64_t count, float scale, hipStream_t stream);

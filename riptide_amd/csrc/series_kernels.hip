@@ -80,6 +80,15 @@ __global__ void downsample_vertical_kernel(const float* __restrict__ x, int64_t 
     }
 }
 
+// 8-bit samples of a staged file to float32 (kind 1: unsigned, 2: signed).
+__global__ void convert_samples_kernel(const uint8_t* __restrict__ raw, int64_t n, int kind,
+                                       float* __restrict__ out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = kind == 2 ? (float)(int8_t)raw[i] : (float)raw[i];
+}
+
 // First output sample whose imin is >= c.
 __device__ __forceinline__ int64_t ds_first_at(int64_t c, const DsStepDev& st)
 {
@@ -695,6 +704,14 @@ hipError_t launch_downsample_vertical(const float* x, int64_t rows, int64_t bins
     split_double(f, &mf, &ef);
     hipLaunchKernelGGL(downsample_vertical_kernel, dim3(grid_for(n * bins, 256)), dim3(256), 0,
                        stream, x, rows, bins, mf, ef, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_samples(const uint8_t* raw, int64_t n, int kind, float* out,
+                                  hipStream_t stream)
+{
+    hipLaunchKernelGGL(convert_samples_kernel, dim3(grid_for(n, 256)), dim3(256), 0, stream, raw,
+                       n, kind, out);
     return hipGetLastError();
 }
 

@@ -52,6 +52,38 @@ class Metadata(dict):
         for k in RESERVED:
             self.setdefault(k, None)
 
+    @classmethod
+    def from_presto_inf(cls, inf):
+        """Metadata of a PRESTO .inf file (metadata.py:53-71); `inf` is a
+        PrestoInf or a path."""
+        import os
+        from .reading.presto import PrestoInf
+        if isinstance(inf, str):
+            inf = PrestoInf(inf)
+        attrs = dict(inf)
+        attrs["skycoord"] = inf.skycoord
+        attrs["fname"] = os.path.realpath(inf.fname)
+        attrs["tobs"] = attrs["tsamp"] * attrs["nsamp"]
+        return cls(attrs)
+
+    @classmethod
+    def from_sigproc(cls, sh, extra_keys=None):
+        """Metadata of a SIGPROC time series file (metadata.py:73-111); `sh`
+        is a SigprocHeader or a path."""
+        import os
+        from .reading.sigproc import SigprocHeader
+        if isinstance(sh, str):
+            sh = SigprocHeader(sh, extra_keys=extra_keys)
+        sh.check_time_series()
+        attrs = dict(sh)
+        attrs["dm"] = attrs.get("refdm", None)
+        attrs["skycoord"] = sh.skycoord
+        attrs["source_name"] = attrs.get("source_name", None)
+        attrs["mjd"] = attrs.get("tstart", None)
+        attrs["fname"] = os.path.realpath(sh.fname)
+        attrs["tobs"] = sh.tobs
+        return cls(attrs)
+
     def to_dict(self):
         return dict(self)
 

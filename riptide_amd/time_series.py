@@ -65,6 +65,35 @@ class TimeSeries(object):
             return TimeSeries(downsample(self.data, factor), factor * self.tsamp,
                               metadata=self.metadata)
 
+    @classmethod
+    def from_presto_inf(cls, fname):
+        """TimeSeries from a PRESTO .inf file and its .dat file
+        (time_series.py:301-333)."""
+        import warnings
+        from .reading.presto import PrestoInf
+        inf = PrestoInf(fname)
+        ts = cls(inf.load_data(), tsamp=inf["tsamp"], metadata=Metadata.from_presto_inf(inf))
+        band = ts.metadata["em_band"]
+        if band in ("X-ray", "Gamma"):
+            warnings.warn(
+                f" You have loaded file {fname!r}, which contains data observed at a "
+                f"high-energy band {band!r}. riptide is NOT designed to process low photon count "
+                "time series, i.e. where the background noise statistics are non-Gaussian. Be "
+                "VERY careful when interpreting any search outputs.", category=UserWarning)
+        return ts
+
+    @classmethod
+    def from_sigproc(cls, fname, extra_keys=None):
+        """TimeSeries from a SIGPROC dedispersed time series file, 8-bit (with
+        a 'signed' header key) or 32-bit (time_series.py:335-377)."""
+        from .reading.sigproc import SigprocHeader
+        sig = SigprocHeader(fname, extra_keys=extra_keys)
+        meta = Metadata.from_sigproc(sig)
+        with open(fname, "rb") as f:
+            f.seek(sig.bytesize)
+            data = np.fromfile(f, dtype=sig.sample_dtype).astype(np.float32)
+        return cls(data, tsamp=sig["tsamp"], metadata=meta)
+
     def fold(self, period, bins, subints=None):
         """Fold at `period` seconds into `bins` phase bins
         (time_series.py:153-174, folding.py:21-85)."""

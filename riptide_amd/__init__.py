@@ -11,6 +11,8 @@ from .search import ffa_search
 from .running_medians import running_median, fast_running_median
 from .libffa import ffa1, ffa2, ffafreq, ffaprd, generate_signal, downsample, boxcar_snr
 from .ffautils import generate_width_trials
+from .peak_detection import Peak, find_peaks
+from .serialization import save_json, load_json
 
 __version__ = "0.1.0"
 
@@ -29,4 +31,8 @@ __all__ = [
     "running_median",
     "fast_running_median",
     "generate_width_trials",
+    "Peak",
+    "find_peaks",
+    "save_json",
+    "load_json",
 ]

@@ -9,7 +9,6 @@ channels, channel width, analyst; X-ray / Gamma: field of view, central
 energy, energy band, analyst).  Sky positions are returned as plain (RA
 hours, Dec degrees) pairs: this package does not depend on astropy."""
 import os
-import typing
 
 import numpy as np
 
@@ -17,11 +16,24 @@ _COLUMN = 40
 _FAKE = "None (Artificial Data Set)"
 
 
-class SkyPosition(typing.NamedTuple):
-    """Sky position: right ascension in hours, declination in degrees."""
+class SkyPosition:
+    """Sky position: right ascension in hours, declination in degrees (a
+    plain object, not a tuple, so that JSON encoding tags it)."""
 
-    ra_hours: float
-    dec_deg: float
+    __slots__ = ("ra_hours", "dec_deg")
+
+    def __init__(self, ra_hours, dec_deg):
+        self.ra_hours = float(ra_hours)
+        self.dec_deg = float(dec_deg)
+
+    def __iter__(self):
+        return iter((self.ra_hours, self.dec_deg))
+
+    def __eq__(self, other):
+        return isinstance(other, SkyPosition) and tuple(self) == tuple(other)
+
+    def __repr__(self):
+        return f"SkyPosition(ra_hours={self.ra_hours!r}, dec_deg={self.dec_deg!r})"
 
 
 def _sexagesimal(text):

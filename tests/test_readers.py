@@ -52,3 +52,28 @@ def test_sigproc_unknown_key_needs_a_type(tmp_path):
         SigprocHeader(str(p))
     h = SigprocHeader(str(p), extra_keys={"mystery": int})
     assert h["mystery"] == 7 and h["nbits"] == 32 and h.bytesize == len(raw)
+
+
+def test_json_round_trip_and_wire_format(tmp_path):
+    """save_json/load_json (serialization.py of the reference): arrays as
+    base64 numpy.ndarray records, sky positions as astropy.SkyCoord
+    records, objects tagged with __type__ / __version__."""
+    import json
+    ts = rt.TimeSeries.from_presto_inf(os.path.join(DATA, "fake_presto_radio.inf"))
+    pg = rt.Periodogram(np.array([1, 2, 3]), np.linspace(1.0, 2.0, 5),
+                        np.arange(240, 245, dtype=np.uint32),
+                        np.arange(15, dtype=np.float32).reshape(5, 3), metadata=ts.metadata)
+    for obj in (ts, pg):
+        p = tmp_path / "x.json"
+        rt.save_json(str(p), obj)
+        raw = json.loads(p.read_text())
+        assert raw["__type__"] == type(obj).__name__ and "__version__" in raw
+        back = rt.load_json(str(p))
+        assert type(back) is type(obj)
+    assert raw["snrs"]["__type__"] == "numpy.ndarray" and raw["snrs"]["dtype"] == "float32"
+    assert raw["metadata"]["skycoord"]["__type__"] == "astropy.SkyCoord"
+    assert abs(raw["metadata"]["skycoord"]["rajd"] - 15.0 / 3600) < 1e-12
+    assert np.array_equal(back.snrs, pg.snrs) and np.array_equal(back.periods, pg.periods)
+    assert back.metadata["dm"] == 42.42
+    t2 = rt.load_json(str(tmp_path / "x.json"))
+    assert t2.metadata["skycoord"] == pg.metadata["skycoord"]

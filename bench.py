@@ -372,9 +372,11 @@ def main():
         peaks_info = peak_timing(torch, batch, outs, bmin, tsamp, args.nsamp)
         log(f"find_peaks: {peaks_info}")
 
-        # plain schedule: one series at a time, per-step launches, no graph
-        nplain = min(args.plain_trials, args.trials)
-        idx = list(range(nplain))
+        # plain schedule: one series at a time, per-step launches, no graph.
+        # Timed per bins class on the first plain_trials/2 series of each
+        # class, then weighted by the batch's class mix, so that both rates
+        # describe the same workload
+        per_class = max(1, min(args.plain_trials, args.trials) // 2)
         widths = {b: generate_width_trials(b) for b in (240, 480)}
         tmp = _lib.empty((args.nsamp,))
         tmp2 = _lib.empty((args.nsamp,))
@@ -383,21 +385,29 @@ def main():
                      for b in (240, 480)}  # warm-up + buffers
         torch.cuda.synchronize()
         ws = int(round(4.0 / tsamp))
-        t0 = time.perf_counter()
-        for i in idx:
-            b0, b1 = int(bmin[i]), int(bmax[i])
-            _lib.call("rt_deredden", _lib.ptr(xs[i]), args.nsamp, ws, 101, _lib.ptr(tmp),
-                      _lib.stream())
-            _lib.call("rt_normalise", _lib.ptr(tmp), args.nsamp, _lib.ptr(tmp2), _lib.stream())
-            P, F, S = plain_out[b0]
-            w = np.ascontiguousarray(widths[b0], dtype=np.uint64)
-            _lib.call("rt_periodogram_plain", _lib.ptr(tmp2), args.nsamp, tsamp,
-                      _lib.host_ptr(w), w.size, pmin, pmax, b0, b1, _lib.ptr(P), _lib.ptr(F),
-                      _lib.ptr(S), _lib.stream())
-        torch.cuda.synchronize()
-        plain_s = time.perf_counter() - t0
-        plain_tps = nplain / plain_s
-        log(f"plain: {nplain} trials in {plain_s:.3f} s -> {plain_tps:.1f} trials/s")
+        plain_ms = {}
+        for cls in (0, 1):
+            idx = [i for i in range(args.trials) if coin[i] == cls][:per_class]
+            t0 = time.perf_counter()
+            for i in idx:
+                b0, b1 = int(bmin[i]), int(bmax[i])
+                _lib.call("rt_deredden", _lib.ptr(xs[i]), args.nsamp, ws, 101, _lib.ptr(tmp),
+                          _lib.stream())
+                _lib.call("rt_normalise", _lib.ptr(tmp), args.nsamp, _lib.ptr(tmp2),
+                          _lib.stream())
+                P, F, S = plain_out[b0]
+                w = np.ascontiguousarray(widths[b0], dtype=np.uint64)
+                _lib.call("rt_periodogram_plain", _lib.ptr(tmp2), args.nsamp, tsamp,
+                          _lib.host_ptr(w), w.size, pmin, pmax, b0, b1, _lib.ptr(P),
+                          _lib.ptr(F), _lib.ptr(S), _lib.stream())
+            torch.cuda.synchronize()
+            plain_ms[cls] = (time.perf_counter() - t0) / max(1, len(idx)) * 1e3
+        frac480 = float((coin == 1).mean())
+        plain_trial_ms = (1 - frac480) * plain_ms[0] + frac480 * plain_ms[1]
+        plain_tps = 1e3 / plain_trial_ms
+        nplain = 2 * per_class
+        log(f"plain: {plain_ms[0]:.2f} ms per 240-260 trial, {plain_ms[1]:.2f} ms per 480-520 "
+            f"trial -> {plain_tps:.1f} trials/s at the batch's class mix")
 
     # heterogeneous batch: 512 series, lengths log-uniform 2^21..2^23 rounded
     # to a multiple of 1024, same noise model and bins coin
@@ -496,7 +506,11 @@ def main():
             "group_size": args.group_size,
         },
         "plain": {"value": round(plain_tps, 2), "unit": "DM trials/s", "trials": nplain,
-                  "schedule": "one series per launch sequence: per-step launches, no queue, no graph"},
+                  "ms_per_trial_by_class": {"240-260": round(plain_ms[0], 3),
+                                            "480-520": round(plain_ms[1], 3)},
+                  "schedule": ("one series per launch sequence: per-step launches, no queue, no "
+                               "graph; timed per bins class and weighted by the batch's class "
+                               "mix")},
         "batched_over_plain": round(trials_per_s / plain_tps, 3),
         "phases_ms": {k: round(v, 3) if isinstance(v, float) else v for k, v in phases.items()},
         "parity": parity,

@@ -148,9 +148,17 @@ static int run_passes(const std::vector<std::vector<FfaTask>>& passes, hipStream
     std::vector<size_t> first;
     for (const auto& p : passes) {
         first.push_back(all.size());
-        std::vector<FfaTask> q = p;
-        sort_by_variant(q);
-        all.insert(all.end(), q.begin(), q.end());
+        const size_t at = all.size();
+        all.insert(all.end(), p.begin(), p.end());
+        // most passes hold one variant only; sort the others in place
+        bool mixed = false;
+        for (size_t i = 1; i < p.size() && !mixed; ++i)
+            mixed = ffa_variant(p[i].bins) != ffa_variant(p[0].bins);
+        if (mixed) {
+            std::vector<FfaTask> q(all.begin() + at, all.end());
+            sort_by_variant(q);
+            std::copy(q.begin(), q.end(), all.begin() + at);
+        }
     }
     if (all.empty()) return RT_OK;
     FfaTask* d_tasks = nullptr;

@@ -37,10 +37,11 @@ class BatchSearch:
         self.period_min = float(period_min)
         self.period_max = float(period_max)
         h = ctypes.c_void_p()
-        # FFA workgroup shape: explicit, else the autotuner's choice for this
-        # bins range (riptide_amd.tune), else the library default
+        # FFA workgroup shape: explicit, else the autotuner's choice for the
+        # widest bins class of the batch (its transforms cost the most;
+        # riptide_amd.tune), else the library default
         if shape is None:
-            shape = tune.shape_for("batch", int(self.bins_min.min()))
+            shape = tune.shape_for("batch", int(self.bins_min.max()))
         self.shape = shape
         if shape is not None:
             _lib.call("rt_set_ffa_shape", int(shape))

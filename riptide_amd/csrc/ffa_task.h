@@ -308,7 +308,8 @@ struct RowCore {
     // of `data` is the -0.0f row, the tail of copied rows.
     static __device__ __forceinline__ int merge(const Seg* seg, float* data, const uint32_t* tab,
                                                 int rows, int levels, bool cone, int bins,
-                                                int pitch, int cap_rows, int wave, int lane)
+                                                int pitch, int cap_rows, int wave, int lane,
+                                                gfloat* gdst)
     {
         const int kd = (bins + kWave - 1) / kWave;
         const bool last_ok = lane + (K - 1) * kWave < bins;
@@ -328,6 +329,7 @@ struct RowCore {
 #pragma unroll
             for (int i0 = 0; i0 < R; i0 += G) {
                 if (i0 < nr) {
+                    float hv[G * K], tv[G * K];
 #pragma unroll
                     for (int g = 0; g < G; ++g) {
                         const int i = i0 + g;
@@ -350,14 +352,39 @@ struct RowCore {
                         for (int q = 0; q < K; ++q) {
                             if (EXACT || q < kd) {
                                 const uint32_t tb = lane + q * kWave >= lim ? t2 : t1;
-                                const float tv = lds_load(tb + q * kWave * 4);
-                                v[i * K + q] = hrow[q * kWave] + tv;
+                                tv[g * K + q] = lds_load(tb + q * kWave * 4);
+                                hv[g * K + q] = hrow[q * kWave];
                             }
                         }
+                    }
+                    // every load of the group is issued before the first add
+                    asm volatile("" ::: "memory");
+#pragma unroll
+                    for (int g = 0; g < G; ++g) {
+                        const int i = i0 + g;
+                        if (i >= R) break;
+#pragma unroll
+                        for (int q = 0; q < K; ++q)
+                            if (EXACT || q < kd) v[i * K + q] = hv[g * K + q] + tv[g * K + q];
                     }
                 }
             }
             __syncthreads();
+            if (t == 0 && gdst != nullptr) {
+                // the last level of a task whose rows go to HBM: written
+                // straight from registers (row p to gdst + p*bins); no LDS
+                // is read after the barrier above, so none is needed after
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    if (i < nr) {
+                        gfloat* grow = gdst + (size_t)(wave + i * NW) * bins + lane;
+#pragma unroll
+                        for (int q = 0; q < K; ++q)
+                            if (col_ok(q, kd, last_ok, lane, bins)) grow[q * kWave] = v[i * K + q];
+                    }
+                }
+                return out_rows;
+            }
 #pragma unroll
             for (int i = 0; i < R; ++i) {
                 if (i < nr) {
@@ -391,11 +418,13 @@ struct RowCore {
         tables();
         load_commit(v, data, nr, bins, pitch, wave, lane);
         __syncthreads();
-        rows = merge(seg, data, tab, rows, __builtin_amdgcn_readfirstlane(tk.levels), tk.cone != 0,
-                     bins, pitch, __builtin_amdgcn_readfirstlane(tk.pad), wave, lane);
-        if (tk.kind == 0) {
-            store(as_global(tk.dst) + ((size_t)tk.node_r0 + tk.lo) * bins, data, rows, bins, pitch,
-                  wave, lane);
+        const int levels = __builtin_amdgcn_readfirstlane(tk.levels);
+        gfloat* dst = tk.kind == 0 ? as_global(tk.dst) + ((size_t)tk.node_r0 + tk.lo) * bins : nullptr;
+        rows = merge(seg, data, tab, rows, levels, tk.cone != 0, bins, pitch,
+                     __builtin_amdgcn_readfirstlane(tk.pad), wave, lane, dst);
+        if (tk.kind == 0 && levels == 0) {
+            // no merge level: the staged rows themselves
+            store(dst, data, rows, bins, pitch, wave, lane);
             __syncthreads();
         }
         return rows;
@@ -494,11 +523,12 @@ __device__ __forceinline__ void window_max_chunk(const float* cps, int i0, int i
 // data + p*pitch), written to tk.snr.  `scratch` is free LDS.
 //  1. circular prefix sums in place (kernels.hpp:73-90): each row is cut
 //     into C chunks; one thread per (chunk, row), chunk-major so that the
-//     lanes of a wavefront take the same chunk of consecutive rows.  Chunk
-//     totals in float-float go to LDS, then every chunk is rescanned from
-//     the float-float sum of the chunks before it, writing each partial sum
-//     rounded once to float32 -- what the reference's float64 accumulator
-//     gives.
+//     lanes of a wavefront take the same chunk of consecutive rows, with C
+//     chosen so that about every thread of the workgroup has an item.
+//     Chunk totals in float-float go to LDS, each row's totals are scanned,
+//     then every chunk is rescanned from the float-float sum of the chunks
+//     before it, writing each partial sum rounded once to float32 -- what
+//     the reference's float64 accumulator gives.
 //  2. window maxima: the same (chunk, row) threads, all widths per thread
 //     (each prefix value read once per width), combined across chunks with
 //     LDS integer max; then one thread per (row, width) writes the S/N.
@@ -514,13 +544,15 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
     const int bins = tk.bins;
     const int nw = tk.nw;
     if (nrows <= 0) return;
-    // (chunk, row) work items: wavefront item wc covers rows 64*(wc % wpc) ..
-    // of chunk wc / wpc; lanes past the last row repeat it and store nothing
-    const int wpc = (nrows + kWave - 1) / kWave;  // wavefronts per chunk
-    int C = max(1, NW / wpc);
+    // (chunk, row) work items, item it = c * nrows + p: every thread takes
+    // one item where C * nrows <= NT, so no lane idles for want of a row.
+    // Chunk c covers [chunk_lo(c), chunk_lo(c + 1)) in whole groups of 4
+    // columns (the last chunk also takes the remainder); the lanes of one
+    // wavefront work on consecutive rows of one or two chunks.
+    int C = max(1, NT / nrows);
     C = min(C, max(1, bins / 8));                      // chunks of >= 8 columns
     C = min(C, max(1, scratch_floats / (2 * nrows)));  // chunk totals fit
-    const int L = (bins + C - 1) / C;
+    const int nitems = C * nrows;
     float* tot = scratch;  // [C][nrows] float-float pairs
     // widths and (h + b, b) pairs of the transform: loaded now, stored to
     // LDS after the first prefix pass, so that their latency is hidden
@@ -531,82 +563,87 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         if (lane < nwl) w_ld = as_global(tk.widths)[lane];
         if (lane < 2 * nwl) hb_ld = as_global(tk.hb)[lane];
     }
-    for (int wc = wave; wc < C * wpc; wc += NW) {
-        const int c = wc / wpc;
-        const int pr = (wc - c * wpc) * kWave + lane;
-        const int p = min(pr, nrows - 1);
+    const int g4 = bins >> 2;
+    // (1) chunk totals in float-float: the running float sum s and the sum
+    // l of its exact rounding errors
+    for (int it = tid; it < nitems; it += NT) {
+        const int c = it / nrows;
+        const int p = it - c * nrows;
         const float* row = data + (size_t)p * pitch;
-        const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
+        int j = 4 * ((c * g4) / C);
+        const int je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
         float sh = 0.0f, sl = 0.0f;
-        int j = j0;
-        for (; j + 4 <= j1; j += 4) {
+        for (; j + 4 <= je; j += 4) {
             float x[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) x[u] = row[j + u];
             asm volatile("" ::: "memory");
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                float s, e;
-                two_sum(sh, x[u], s, e);
+                float t, e;
+                two_sum(sh, x[u], t, e);
+                sh = t;
                 sl += e;
-                sh = s + sl;
-                sl -= sh - s;
             }
         }
-        for (; j < j1; ++j) {
-            float s, e;
-            two_sum(sh, row[j], s, e);
+        for (; j < je; ++j) {
+            float t, e;
+            two_sum(sh, row[j], t, e);
+            sh = t;
             sl += e;
-            sh = s + sl;
-            sl -= sh - s;
         }
-        if (pr < nrows) {
-            tot[2 * (c * nrows + p)] = sh;
-            tot[2 * (c * nrows + p) + 1] = sl;
-        }
+        tot[2 * it] = sh;
+        tot[2 * it + 1] = sl;
     }
     if (wave == NW - 1) {
         if (lane < nwl) s_w[lane] = w_ld;
         if (lane < 2 * nwl) s_hb[lane] = hb_ld;
     }
     __syncthreads();
-    for (int wc = wave; wc < C * wpc; wc += NW) {
-        const int c = wc / wpc;
-        const int pr = (wc - c * wpc) * kWave + lane;
-        const int p = min(pr, nrows - 1);
+    // (2) exclusive scan of each row's chunk totals, in place
+    for (int p = tid; p < nrows; p += NT) {
+        float oh = 0.0f, ol = 0.0f;
+        for (int c = 0; c < C; ++c) {
+            const float th = tot[2 * (c * nrows + p)];
+            const float tl = tot[2 * (c * nrows + p) + 1];
+            tot[2 * (c * nrows + p)] = oh;
+            tot[2 * (c * nrows + p) + 1] = ol;
+            ff_add(oh, ol, th, tl, oh, ol);
+        }
+    }
+    __syncthreads();
+    // (3) every chunk rescanned from the sum of the chunks before it, each
+    // partial sum rounded once to float32 -- what the reference's float64
+    // accumulator gives
+    for (int it = tid; it < nitems; it += NT) {
+        const int c = it / nrows;
+        const int p = it - c * nrows;
         float* row = data + (size_t)p * pitch;
-        float sh = 0.0f, sl = 0.0f;
-        for (int cc = 0; cc < c; ++cc)
-            ff_add(sh, sl, tot[2 * (cc * nrows + p)], tot[2 * (cc * nrows + p) + 1], sh, sl);
-        const int j0 = min(bins, c * L), j1 = min(bins, j0 + L);
-        const bool own = pr < nrows;
-        int j = j0;
-        for (; j + 4 <= j1; j += 4) {
+        int j = 4 * ((c * g4) / C);
+        const int je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
+        float sh = tot[2 * it], sl = tot[2 * it + 1];
+        for (; j + 4 <= je; j += 4) {
             float x[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) x[u] = row[j + u];
             asm volatile("" ::: "memory");
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                float s, e;
-                two_sum(sh, x[u], s, e);
+                float t, e;
+                two_sum(sh, x[u], t, e);
+                sh = t;
                 sl += e;
-                sh = s + sl;  // rounded partial sum
-                sl -= sh - s;
-                x[u] = sh;
+                x[u] = sh + sl;  // rounded partial sum
             }
-            if (own) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) row[j + u] = x[u];
-            }
+            for (int u = 0; u < 4; ++u) row[j + u] = x[u];
         }
-        for (; j < j1; ++j) {
-            float s, e;
-            two_sum(sh, row[j], s, e);
+        for (; j < je; ++j) {
+            float t, e;
+            two_sum(sh, row[j], t, e);
+            sh = t;
             sl += e;
-            sh = s + sl;  // rounded partial sum
-            sl -= sh - s;
-            if (own) row[j] = sh;
+            row[j] = sh + sl;
         }
     }
     __syncthreads();
@@ -625,15 +662,16 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         const int wmax = __builtin_amdgcn_readfirstlane(s_w[nw - 1]);
         __syncthreads();
         // (a) window starts [0, bins - wmax), whose windows all stay inside
-        // the row, in C chunks of equal cost, all widths per thread
+        // the row, cut into the same C chunks of whole groups of 4, all
+        // widths per thread
         const int P = max(0, bins - wmax);
-        const int Lw = (P + C - 1) / C;
-        for (int wc = wave; wc < C * wpc; wc += NW) {
-            const int c = wc / wpc;
-            const int pr = (wc - c * wpc) * kWave + lane;
-            const int p = min(pr, nrows - 1);
+        const int q4 = P >> 2;
+        for (int it = tid; it < nitems; it += NT) {
+            const int c = it / nrows;
+            const int p = it - c * nrows;
             const float* cps = data + (size_t)p * pitch;
-            const int i0 = min(P, c * Lw), i1 = min(P, i0 + Lw);
+            const int i0 = 4 * ((c * q4) / C);
+            const int i1 = c == C - 1 ? P : 4 * (((c + 1) * q4) / C);
             if (i0 >= i1) continue;
             float m[kSnrMaxWidths];
 #pragma unroll
@@ -647,11 +685,9 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
 #undef RT_WIN
                 default: break;
             }
-            if (pr < nrows) {
 #pragma unroll
-                for (int j = 0; j < kSnrMaxWidths; ++j)
-                    if (j < nw) atomicMax(&mx[p * nw + j], ordered_int(m[j]));
-            }
+            for (int j = 0; j < kSnrMaxWidths; ++j)
+                if (j < nw) atomicMax(&mx[p * nw + j], ordered_int(m[j]));
         }
         // (b) the last wmax window starts, one thread per (row, width),
         // width-major: windows inside the row, then the ones running past its
@@ -875,10 +911,11 @@ __device__ __forceinline__ bool tree_node(int m, int d, int i, int* r0, int* nm)
 // Next work item of a transform for the persistent kernel, in pass order:
 // pass 0 = whole nodes of depth depth[0] into buf_a, pass p >= 1 = cone tiles
 // of the nodes of depth depth[p], alternating buf_a -> buf_b -> buf_a ...,
-// the last pass producing S/N.  (p, i, lo) is the cursor.
+// the last pass producing S/N.  (p, i, lo) is the cursor; (node, r0, nm)
+// caches the rows of node `node` of pass p (nm = 0: a slot under a leaf).
 __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int32_t* depth,
-                                              int& p, int& i, int& lo, float* buf_a,
-                                              float* buf_b, FfaTask& tk)
+                                              int& p, int& i, int& lo, int& node, int& r0,
+                                              int& nm, float* buf_a, float* buf_b, FfaTask& tk)
 {
     for (;;) {
         if (p >= tr.npass) return false;
@@ -899,10 +936,14 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
             ++p;
             i = 0;
             lo = 0;
+            node = -1;
             continue;
         }
-        int r0, nm;
-        if (!tree_node(tr.rows, d, i, &r0, &nm)) {
+        if (node != i) {
+            if (!tree_node(tr.rows, d, i, &r0, &nm)) nm = 0;
+            node = i;
+        }
+        if (nm == 0) {
             ++i;
             lo = 0;
             continue;
@@ -975,11 +1016,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) 
         tk.rows_eval = tr.rows_eval;
         tk.snr = tr.snr;
         tk.pad = tr.cap;
-        int p = 0, i = 0, lo = 0;
+        int p = 0, i = 0, lo = 0, node = -1, r0 = 0, nm = 0;
         // the pass depths are indexed dynamically: read them from memory, so
         // that the descriptor itself stays in registers
         const int32_t* depth = trs[item].depth;
-        while (next_ffa_task(tr, depth, p, i, lo, buf_a, buf_b, tk))
+        while (next_ffa_task(tr, depth, p, i, lo, node, r0, nm, buf_a, buf_b, tk))
             run_ffa_task<NT, STAGE, V>(tk, lds, err);
     }
 }

@@ -241,6 +241,12 @@ def peak_timing(torch, batch, outs, bmin, tsamp, nsamp, ntrials=16, ncpu=2):
         if key not in finders:
             finders[key] = PeakFinder(P, F, batch.widths[i], tobs)
     t_plan = time.perf_counter() - t0
+    # one untimed call per finder (first-call set-up: buffers, kernel loads)
+    warmed = set()
+    for i in range(ntrials):
+        if int(bmin[i]) not in warmed:
+            finders[int(bmin[i])].find(outs[i])
+            warmed.add(int(bmin[i]))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(ntrials):

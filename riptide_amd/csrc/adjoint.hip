@@ -173,7 +173,7 @@ __global__ void downsample_index_kernel(int64_t n, uint64_t mf, int ef, int64_t 
 {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * blockDim.x) {
-        const DsIndex ix = ds_index(k, mf, ef, N);
+        const DsIndex ix = ds_index_fast(k, mf, ef, N);
         idx[k] = make_int4((int)ix.imin, (int)ix.imax, __float_as_int(ix.wmin),
                            __float_as_int(ix.wmax));
     }

@@ -254,7 +254,7 @@ int rt_downsample_index(int64_t k, double f, int64_t size, int64_t* imin, int64_
     uint64_t mf;
     int ef;
     split_double(f, &mf, &ef);
-    const DsIndex ix = ds_index(k, mf, ef, size);
+    const DsIndex ix = ds_index_fast(k, mf, ef, size);
     *imin = ix.imin;
     *imax = ix.imax;
     *wmin = ix.wmin;

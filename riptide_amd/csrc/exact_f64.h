@@ -135,10 +135,87 @@ __host__ __device__ inline DsIndex ds_index(int64_t k, uint64_t mf, int ef, int6
     return r;
 }
 
+// The same index math with 64-bit integer operations, for the factors of a
+// search (1 < f < 2^11, so -52 <= ef <= -43) and k < 2^40: every value is a
+// fixed-point number with F = -ef fraction bits.  fl(k*f) rounds the 128-bit
+// product k*mf at bit sh = bitlen - 53 (< 64), which only moves bits below the
+// binary point (sh < F) and carries into the integer part when the fraction
+// overflows; fl(fl(k*f) + f) likewise.  Falls back to ds_index outside that
+// range.
+__host__ __device__ inline int bitlen64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }
+
+__host__ __device__ inline uint64_t mul64hi(uint64_t a, uint64_t b)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umul64hi(a, b);
+#else
+    return (uint64_t)(((u128)a * b) >> 64);
+#endif
+}
+
+// Rounds the fixed-point value (hi:lo) to 53 significant bits (RNE) in place.
+__host__ __device__ inline void round53_fixed(uint64_t& hi, uint64_t& lo)
+{
+    const int nb = hi ? 64 + bitlen64(hi) : bitlen64(lo);
+    if (nb <= 53) return;
+    const int sh = nb - 53;  // < 64 for the ranges above
+    const uint64_t mask = (((uint64_t)1) << sh) - 1;
+    const uint64_t rem = lo & mask;
+    const uint64_t half = ((uint64_t)1) << (sh - 1);
+    const bool up = rem > half || (rem == half && ((lo >> sh) & 1));
+    lo -= rem;
+    if (up) {
+        const uint64_t add = ((uint64_t)1) << sh;
+        lo += add;
+        if (lo < add) ++hi;
+    }
+}
+
+__host__ __device__ inline DsIndex ds_index_fast(int64_t k, uint64_t mf, int ef, int64_t N)
+{
+    const int F = -ef;
+    if (F < 43 || F > 52 || k < 0 || k >= ((int64_t)1 << 40)) return ds_index(k, mf, ef, N);
+    const uint64_t fmask = (((uint64_t)1) << F) - 1;
+    // start = fl(k * f)
+    uint64_t hi = mul64hi((uint64_t)k, mf);
+    uint64_t lo = (uint64_t)k * mf;
+    round53_fixed(hi, lo);
+    const int64_t i0 = (int64_t)((hi << (64 - F)) | (lo >> F));
+    const uint64_t fr0 = lo & fmask;
+    // end = fl(start + f)
+    uint64_t lo2 = lo + mf;
+    uint64_t hi2 = hi + (lo2 < mf ? 1 : 0);
+    round53_fixed(hi2, lo2);
+    const int64_t i1 = (int64_t)((hi2 << (64 - F)) | (lo2 >> F));
+    const uint64_t fr1 = lo2 & fmask;
+    DsIndex r;
+    r.imin = i0;
+    r.imax = i1 < N - 1 ? i1 : N - 1;
+    // (imin + 1) - start = (2^F - fr0) * 2^-F, exact in float64
+    r.wmin = ldexpf((float)((fmask + 1) - fr0), -F);
+    if (r.imax == i1) {
+        r.wmax = ldexpf((float)fr1, -F);
+    } else {
+        // end - imax = (i1 - imax) + fr1 * 2^-F
+        const uint64_t whole = (uint64_t)(i1 - r.imax);
+        if (whole < (((uint64_t)1) << (63 - F)))
+            r.wmax = ldexpf((float)((whole << F) | fr1), -F);
+        else
+            r.wmax = ds_index(k, mf, ef, N).wmax;
+    }
+    return r;
+}
+
 // imin(k) = floor(fl(k*f))
 __host__ __device__ inline int64_t ds_imin(int64_t k, uint64_t mf, int ef)
 {
-    return dfloor(dmul_int((uint64_t)k, mf, ef));
+    const int F = -ef;
+    if (F < 43 || F > 52 || k < 0 || k >= ((int64_t)1 << 40))
+        return dfloor(dmul_int((uint64_t)k, mf, ef));
+    uint64_t hi = mul64hi((uint64_t)k, mf);
+    uint64_t lo = (uint64_t)k * mf;
+    round53_fixed(hi, lo);
+    return (int64_t)((hi << (64 - F)) | (lo >> F));
 }
 
 }  // namespace rt

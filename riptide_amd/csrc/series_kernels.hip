@@ -34,7 +34,7 @@ __global__ void downsample_direct_kernel(const float* __restrict__ in, int64_t N
 {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
          k += (int64_t)gridDim.x * blockDim.x) {
-        const DsIndex ix = ds_index(k, mf, ef, N);
+        const DsIndex ix = ds_index_fast(k, mf, ef, N);
         out[k] = ds_accumulate(in, ix);
     }
 }
@@ -72,7 +72,7 @@ __global__ void downsample_vertical_kernel(const float* __restrict__ x, int64_t 
          e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t k = e / bins;
         const int64_t b = e - k * bins;
-        const DsIndex ix = ds_index(k, mf, ef, rows);
+        const DsIndex ix = ds_index_fast(k, mf, ef, rows);
         const float* col = x + b;
         float acc = ix.wmin * col[(size_t)ix.imin * bins];
         for (int64_t i = ix.imin + 1; i < ix.imax; ++i) acc += col[(size_t)i * bins];
@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
         }
         const DsStepDev& p = st[a];
         const int64_t k = s_kbeg[a] + (e - s_cnt_prefix[a]);
-        DsIndex ix = ds_index(k, p.mf, p.ef, sr.N);
+        DsIndex ix = ds_index_fast(k, p.mf, p.ef, sr.N);
         ix.imin -= c0;
         ix.imax -= c0;
         sr.out[p.out_offset + k] = ds_accumulate(s_in, ix);

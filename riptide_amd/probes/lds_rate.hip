@@ -73,6 +73,42 @@ void run(const char* name, float* out, long long* cyc)
     const double wave_loads = 16.0 * iters * 8;
     printf("%-18s W=%d: %.0f cycles, %.1f B/clk/CU, %.2f clk per wave-load\n", name, W, avg, bytes / avg, avg / wave_loads);
 }
+// The access pattern of one FFA merge level without its barriers: per
+// element a 4-byte read of a head row, a 4-byte read of a tail row rotated by
+// `sh` columns (wrap chosen per element by a compare and a select), their sum
+// written back over the head row; lanes on consecutive columns, rows of 512
+// floats, each wavefront on its own pair of rows (1024 floats of s[16384]).
+__global__ __launch_bounds__(1024) void lds_merge(float* out, long long* cycles, int iters)
+{
+    __shared__ float s[16384];
+    for (int i = threadIdx.x; i < 16384; i += 1024) s[i] = (float)(i & 255);
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    float* head = s + wave * 1024;
+    const float* tail = head + 512;
+    long long t0 = clock64();
+    for (int it = 0; it < iters; ++it) {
+        const int sh = (it * 37) & 511;
+        const int lim = 512 - sh;
+        float v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int c = lane + 64 * q;
+            const int t = c >= lim ? c + sh - 512 : c + sh;
+            v[q] = head[c] + tail[t];
+        }
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int q = 0; q < 8; ++q) head[lane + 64 * q] = v[q];
+        asm volatile("" ::: "memory");
+    }
+    long long t1 = clock64();
+    __syncthreads();
+    if (threadIdx.x == 0) cycles[blockIdx.x] = t1 - t0;
+    out[blockIdx.x * 1024 + threadIdx.x] = head[lane];
+}
+
 int main()
 {
     float* out; long long* cyc;
@@ -89,6 +125,19 @@ int main()
     run<4, 1, 236>("rows pitch 236", out, cyc);
     run<4, 1, 228>("rows pitch 228", out, cyc);
     run<4, 1, 240>("rows pitch 240", out, cyc);
+    {
+        // every index above stays inside s[16384]: wave*1024 + 512 + 511
+        const int blocks = 256, iters = 4096;
+        long long h[256];
+        for (int rep = 0; rep < 2; ++rep) {
+            hipLaunchKernelGGL(lds_merge, dim3(blocks), dim3(1024), 0, 0, out, cyc, iters);
+            (void)hipDeviceSynchronize();
+        }
+        (void)hipMemcpy(h, cyc, sizeof(h), hipMemcpyDeviceToHost);
+        double avg = 0; for (int i = 0; i < blocks; ++i) avg += h[i]; avg /= blocks;
+        const double bytes = 1024.0 * iters * 8 * 3 * 4;  // two reads + one write per element
+        printf("%-18s    : %.0f cycles, %.1f B/clk/CU (2 reads + 1 write per element)\n", "merge pattern", avg, bytes / avg);
+    }
     printf("done\n");
     return 0;
 }

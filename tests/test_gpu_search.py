@@ -136,3 +136,32 @@ def test_ffa_search_returns_same_object():
                              already_normalised=True, deredden=False)
     assert id(tsdr) == id(ts)
     assert all(np.maximum.accumulate(pg.periods) == pg.periods)
+
+
+# Tiles of many short rows (every lane of the S/N phase gets a (chunk, row)
+# item from several chunks per row) and width lists other than the default
+# trials (the S/N phase's generic window path), under every workgroup shape.
+SHAPE_CASES = [
+    # N, tsamp, pmin, pmax, bmin, bmax, widths (None: default trials)
+    (2 ** 16, 1e-3, 0.02, 0.2, 16, 20, None),
+    (2 ** 16, 1e-3, 0.05, 0.5, 40, 44, None),
+    (2 ** 17, 256e-6, 0.5, 2.0, 240, 260, [1, 3, 5, 8, 21, 40]),
+    (2 ** 17, 256e-6, 0.5, 2.0, 480, 520, [2, 7]),
+]
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("case", SHAPE_CASES)
+def test_periodogram_parity_shapes(oracle, case, shape):
+    from riptide_amd import libcpp, tune
+    N, tsamp, pmin, pmax, bmin, bmax, widths = case
+    x = oracle.normalise(_pulsar(N, tsamp, 0.37 * (pmin + pmax), 0.05, 15.0, seed=bmin + shape))
+    w = oracle.generate_width_trials(bmin) if widths is None else np.asarray(widths)
+    tune.set_shape(shape)
+    try:
+        P, F, S = libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    finally:
+        tune.set_shape(None)
+    Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    assert np.array_equal(P, Po) and np.array_equal(F, Fo)
+    np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)

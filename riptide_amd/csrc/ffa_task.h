@@ -370,10 +370,13 @@ struct RowCore {
                 }
             }
             __syncthreads();
-            if (t == 0 && gdst != nullptr) {
+            if (K >= 6 && t == 0 && gdst != nullptr) {
                 // the last level of a task whose rows go to HBM: written
                 // straight from registers (row p to gdst + p*bins); no LDS
-                // is read after the barrier above, so none is needed after
+                // is read after the barrier above, so none is needed after.
+                // Rows of fewer than 6 columns per lane keep the LDS store
+                // phase: their many rows per wavefront would need more
+                // address registers than the kernel has.
 #pragma unroll
                 for (int i = 0; i < R; ++i) {
                     if (i < nr) {
@@ -422,8 +425,9 @@ struct RowCore {
         gfloat* dst = tk.kind == 0 ? as_global(tk.dst) + ((size_t)tk.node_r0 + tk.lo) * bins : nullptr;
         rows = merge(seg, data, tab, rows, levels, tk.cone != 0, bins, pitch,
                      __builtin_amdgcn_readfirstlane(tk.pad), wave, lane, dst);
-        if (tk.kind == 0 && levels == 0) {
-            // no merge level: the staged rows themselves
+        if (tk.kind == 0 && (levels == 0 || K < 6)) {
+            // no merge level (the staged rows themselves), or the last level
+            // was written back to LDS
             store(dst, data, rows, bins, pitch, wave, lane);
             __syncthreads();
         }

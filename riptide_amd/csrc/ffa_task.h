@@ -301,11 +301,14 @@ struct RowCore {
         }
     }
 
-    // All merge levels, bottom-up, in place.  Per element: two LDS reads
-    // (head column c, tail column (c + shift) mod bins, the wrap chosen by
-    // one compare against a per-row scalar) and one add; the results stay in
-    // registers until the barrier, then overwrite the level in place.  Row -1
-    // of `data` is the -0.0f row, the tail of copied rows.
+    // All merge levels, bottom-up, in place.  Per element of a single level:
+    // two LDS reads (head column c, tail column (c + shift) mod bins, the wrap
+    // chosen by one compare against a per-row scalar) and one add; the
+    // results stay in registers until the barrier, then overwrite the level
+    // in place.  Rows of K >= 6 columns per lane take two levels per pass
+    // (four reads and three adds per element, the row tables of both levels
+    // composed per row): half the barriers and table rounds, 5/6 of the LDS
+    // traffic.  Row -1 of `data` is the -0.0f row, the tail of copied rows.
     static __device__ __forceinline__ int merge(const Seg* seg, float* data, const uint32_t* tab,
                                                 int rows, int levels, bool cone, int bins,
                                                 int pitch, int cap_rows, int wave, int lane,
@@ -313,64 +316,118 @@ struct RowCore {
     {
         const int kd = (bins + kWave - 1) / kWave;
         const bool last_ok = lane + (K - 1) * kWave < bins;
-        for (int t = levels - 1; t >= 0; --t) {
-            const int out_rows = level_rows(cone, seg, t, rows);
-            const uint32_t* tl = tab + (size_t)t * cap_rows;
+        for (int t = levels - 1; t >= 0;) {
+            // levels t and t - 1 in one pass over LDS where two remain
+            const bool two = K >= 6 && t >= 1;
+            const int to = two ? t - 1 : t;  // the level this pass writes
+            const int out_rows = level_rows(cone, seg, to, rows);
+            const uint32_t* tl = tab + (size_t)to * cap_rows;
             const int nr = (out_rows - wave + NW - 1) / NW;
             // this wavefront's row-table entries, all read before the first use
             uint32_t ent[R];
 #pragma unroll
             for (int i = 0; i < R; ++i) ent[i] = tl[min(wave + i * NW, out_rows - 1)];
             float v[R * K];
-            // rows in groups of G without a branch between them, so that the
-            // loads of a whole group are in flight together (a group may
-            // run past this wavefront's last row: its entries are clamped to
-            // a real row and its results are never written)
+            if (two) {
+                // output row p of level t-1 = X[a] + roll(X[b], sh) with X
+                // = level t, X[r] = A[h(r)] + roll(A[t(r)], sh_r) and A =
+                // level t+1 (in LDS), so
+                //   out = (A[ha] + roll(A[ta], sha)) + (roll(A[hb], sh) + roll(A[tb], sh + shb))
+                // in the reference's order of additions; a copied row (b =
+                // -1) takes the -0.0 row for both terms of X[b]
+                const uint32_t* tm = tab + (size_t)t * cap_rows;
 #pragma unroll
-            for (int i0 = 0; i0 < R; i0 += G) {
-                if (i0 < nr) {
-                    float hv[G * K], tv[G * K];
-#pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const int i = i0 + g;
-                        if (i >= R) break;
+                for (int i = 0; i < R; ++i) {
+                    if (i < nr) {
                         const uint32_t e = __builtin_amdgcn_readfirstlane(ent[i]);
                         const int a = (int)(e & 1023u);
-                        const int bt = (int)((e >> 10) & 1023u) - 1;
+                        const int b = (int)((e >> 10) & 1023u) - 1;
                         const int sh = (int)(e >> 20);
-                        const float* hrow = data + a * pitch + lane;
-                        // LDS addresses of the tail column c + sh before and
-                        // after the wrap; the second is hidden from the
-                        // optimiser so that each element costs one compare
-                        // and one select, the column step staying an
-                        // immediate offset
-                        const uint32_t t1 = lds_address(data + bt * pitch + lane + sh);
-                        uint32_t t2 = t1 - bins * 4;
-                        asm volatile("" : "+v"(t2));
-                        const int lim = bins - sh;
+                        const uint32_t fa = __builtin_amdgcn_readfirstlane(tm[a]);
+                        const uint32_t fb = __builtin_amdgcn_readfirstlane(tm[b >= 0 ? b : 0]);
+                        const int ha = (int)(fa & 1023u);
+                        const int ta = (int)((fa >> 10) & 1023u) - 1;
+                        const int sha = (int)(fa >> 20);
+                        const int hb = b >= 0 ? (int)(fb & 1023u) : -1;
+                        const int tb = b >= 0 ? (int)((fb >> 10) & 1023u) - 1 : -1;
+                        const int shb = b >= 0 ? (int)(fb >> 20) : 0;
+                        int shc = sh + shb;
+                        if (shc >= bins) shc -= bins;
+                        const float* hrow = data + ha * pitch + lane;
+                        const uint32_t a1 = lds_address(data + ta * pitch + lane + sha);
+                        const uint32_t b1 = lds_address(data + hb * pitch + lane + sh);
+                        const uint32_t c1 = lds_address(data + tb * pitch + lane + shc);
+                        uint32_t a2 = a1 - bins * 4, b2 = b1 - bins * 4, c2 = c1 - bins * 4;
+                        asm volatile("" : "+v"(a2), "+v"(b2), "+v"(c2));
+                        const int lima = bins - sha, limb = bins - sh, limc = bins - shc;
+                        float x0[K], x1[K], x2[K], x3[K];
 #pragma unroll
                         for (int q = 0; q < K; ++q) {
                             if (EXACT || q < kd) {
-                                const uint32_t tb = lane + q * kWave >= lim ? t2 : t1;
-                                tv[g * K + q] = lds_load(tb + q * kWave * 4);
-                                hv[g * K + q] = hrow[q * kWave];
+                                const int c = lane + q * kWave;
+                                x0[q] = hrow[q * kWave];
+                                x1[q] = lds_load((c >= lima ? a2 : a1) + q * kWave * 4);
+                                x2[q] = lds_load((c >= limb ? b2 : b1) + q * kWave * 4);
+                                x3[q] = lds_load((c >= limc ? c2 : c1) + q * kWave * 4);
                             }
                         }
-                    }
-                    // every load of the group is issued before the first add
-                    asm volatile("" ::: "memory");
-#pragma unroll
-                    for (int g = 0; g < G; ++g) {
-                        const int i = i0 + g;
-                        if (i >= R) break;
+                        asm volatile("" ::: "memory");
 #pragma unroll
                         for (int q = 0; q < K; ++q)
-                            if (EXACT || q < kd) v[i * K + q] = hv[g * K + q] + tv[g * K + q];
+                            if (EXACT || q < kd) v[i * K + q] = (x0[q] + x1[q]) + (x2[q] + x3[q]);
+                    }
+                }
+            } else {
+                // rows in groups of G without a branch between them, so that the
+                // loads of a whole group are in flight together (a group may
+                // run past this wavefront's last row: its entries are clamped to
+                // a real row and its results are never written)
+#pragma unroll
+                for (int i0 = 0; i0 < R; i0 += G) {
+                    if (i0 < nr) {
+                        float hv[G * K], tv[G * K];
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const int i = i0 + g;
+                            if (i >= R) break;
+                            const uint32_t e = __builtin_amdgcn_readfirstlane(ent[i]);
+                            const int a = (int)(e & 1023u);
+                            const int bt = (int)((e >> 10) & 1023u) - 1;
+                            const int sh = (int)(e >> 20);
+                            const float* hrow = data + a * pitch + lane;
+                            // LDS addresses of the tail column c + sh before and
+                            // after the wrap; the second is hidden from the
+                            // optimiser so that each element costs one compare
+                            // and one select, the column step staying an
+                            // immediate offset
+                            const uint32_t t1 = lds_address(data + bt * pitch + lane + sh);
+                            uint32_t t2 = t1 - bins * 4;
+                            asm volatile("" : "+v"(t2));
+                            const int lim = bins - sh;
+#pragma unroll
+                            for (int q = 0; q < K; ++q) {
+                                if (EXACT || q < kd) {
+                                    const uint32_t tb = lane + q * kWave >= lim ? t2 : t1;
+                                    tv[g * K + q] = lds_load(tb + q * kWave * 4);
+                                    hv[g * K + q] = hrow[q * kWave];
+                                }
+                            }
+                        }
+                        // every load of the group is issued before the first add
+                        asm volatile("" ::: "memory");
+#pragma unroll
+                        for (int g = 0; g < G; ++g) {
+                            const int i = i0 + g;
+                            if (i >= R) break;
+#pragma unroll
+                            for (int q = 0; q < K; ++q)
+                                if (EXACT || q < kd) v[i * K + q] = hv[g * K + q] + tv[g * K + q];
+                        }
                     }
                 }
             }
             __syncthreads();
-            if (K >= 6 && t == 0 && gdst != nullptr) {
+            if (K >= 6 && to == 0 && gdst != nullptr) {
                 // the last level of a task whose rows go to HBM: written
                 // straight from registers (row p to gdst + p*bins); no LDS
                 // is read after the barrier above, so none is needed after.
@@ -399,6 +456,7 @@ struct RowCore {
             }
             __syncthreads();
             rows = out_rows;
+            t = to - 1;
         }
         return rows;
     }

@@ -136,10 +136,10 @@ __host__ __device__ inline DsIndex ds_index(int64_t k, uint64_t mf, int ef, int6
 }
 
 // The same index math with 64-bit integer operations, for the factors of a
-// search (1 < f < 2^11, so -52 <= ef <= -43) and k < 2^40: every value is a
+// search (1 <= f < 2^10, so -52 <= ef <= -43) and k < 2^40: every value is a
 // fixed-point number with F = -ef fraction bits.  fl(k*f) rounds the 128-bit
-// product k*mf at bit sh = bitlen - 53 (< 64), which only moves bits below the
-// binary point (sh < F) and carries into the integer part when the fraction
+// product k*mf at bit sh = bitlen - 53 (<= 40 < F), which only moves bits
+// below the binary point and carries into the integer part when the fraction
 // overflows; fl(fl(k*f) + f) likewise.  Falls back to ds_index outside that
 // range.
 __host__ __device__ inline int bitlen64(uint64_t x) { return x ? 64 - __builtin_clzll(x) : 0; }

@@ -41,6 +41,7 @@ def _ds_index_py(k, f, N):
 def test_downsample_index_exact(L):
     rng = np.random.default_rng(0)
     fs = [1.5, 2.0, 3.0, 6.510416666666667, 13.777, 600.3, 4999.5, np.pi, 1.0000001,
+          1023.9999999, 1024.0, 1024.0000001, np.nextafter(2.0, 1.0), 511.75,
           *rng.uniform(1.0001, 700, 40).tolist()]
     imin = ctypes.c_int64()
     imax = ctypes.c_int64()

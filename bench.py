@@ -310,6 +310,7 @@ def main():
                           "bytes_written": 4 * n, "launches": 5}), flush=True)
         return
 
+    from riptide_amd import tune
     from riptide_amd.batch import BatchSearch
     from riptide_amd.ffautils import generate_width_trials
 
@@ -392,9 +393,9 @@ def main():
         torch.cuda.synchronize()
         ws = int(round(4.0 / tsamp))
         plain_ms = {}
-        for cls in (0, 1):
-            idx = [i for i in range(args.trials) if coin[i] == cls][:per_class]
-            t0 = time.perf_counter()
+        plain_shape = {}
+
+        def plain_run(idx):
             for i in idx:
                 b0, b1 = int(bmin[i]), int(bmax[i])
                 _lib.call("rt_deredden", _lib.ptr(xs[i]), args.nsamp, ws, 101, _lib.ptr(tmp),
@@ -406,8 +407,24 @@ def main():
                 _lib.call("rt_periodogram_plain", _lib.ptr(tmp2), args.nsamp, tsamp,
                           _lib.host_ptr(w), w.size, pmin, pmax, b0, b1, _lib.ptr(P),
                           _lib.ptr(F), _lib.ptr(S), _lib.stream())
-            torch.cuda.synchronize()
-            plain_ms[cls] = (time.perf_counter() - t0) / max(1, len(idx)) * 1e3
+
+        # the plain schedule gets its best FFA workgroup shape per class
+        # (one untimed series per shape first)
+        for cls in (0, 1):
+            idx = [i for i in range(args.trials) if coin[i] == cls][:per_class]
+            best = None
+            for shape in range(tune.NUM_SHAPES):
+                _lib.call("rt_set_ffa_shape", shape)
+                plain_run(idx[:1])
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                plain_run(idx)
+                torch.cuda.synchronize()
+                ms = (time.perf_counter() - t0) / max(1, len(idx)) * 1e3
+                if best is None or ms < best[0]:
+                    best = (ms, shape)
+            _lib.call("rt_set_ffa_shape", -1)
+            plain_ms[cls], plain_shape[cls] = best
         frac480 = float((coin == 1).mean())
         plain_trial_ms = (1 - frac480) * plain_ms[0] + frac480 * plain_ms[1]
         plain_tps = 1e3 / plain_trial_ms
@@ -514,9 +531,11 @@ def main():
         "plain": {"value": round(plain_tps, 2), "unit": "DM trials/s", "trials": nplain,
                   "ms_per_trial_by_class": {"240-260": round(plain_ms[0], 3),
                                             "480-520": round(plain_ms[1], 3)},
+                  "workgroup_shape_by_class": {"240-260": plain_shape[0],
+                                               "480-520": plain_shape[1]},
                   "schedule": ("one series per launch sequence: per-step launches, no queue, no "
-                               "graph; timed per bins class and weighted by the batch's class "
-                               "mix")},
+                               "graph; timed per bins class under every FFA workgroup shape, "
+                               "the fastest kept, and weighted by the batch's class mix")},
         "batched_over_plain": round(trials_per_s / plain_tps, 3),
         "phases_ms": {k: round(v, 3) if isinstance(v, float) else v for k, v in phases.items()},
         "parity": parity,

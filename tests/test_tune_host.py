@@ -48,3 +48,15 @@ def test_forced_shape_wins(tmp_path, monkeypatch):
         tune._cache = None
     assert tune.shape_for("batch", 240) == 0
     tune._cache = None
+
+
+def test_command_line_runs_package_module(monkeypatch):
+    """`python -m riptide_amd.tune` must act on the package's module, whose
+    forced shape libcpp reads (a separate `__main__` copy would force a shape
+    nobody reads, and every one-series timing would use the table's shape)."""
+    import runpy
+    from riptide_amd import tune
+    calls = []
+    monkeypatch.setattr(tune, "main", lambda argv=None: calls.append(argv))
+    runpy.run_module("riptide_amd.tune", run_name="__main__")
+    assert calls == [None]

@@ -581,6 +581,51 @@ __device__ __forceinline__ void window_max_chunk(const float* cps, int i0, int i
     }
 }
 
+// Window maxima of the first NWT default widths (kDefaultWidths) over the
+// starts [i0, i1): the same maxima as window_max_chunk, with compile-time
+// offsets.  The small widths (1, 2, 3, 4, 6, 9, 13) of four consecutive
+// starts read only prefix values i .. i + 16, so those are loaded once as one
+// block for the start value and all of them; the larger widths load their
+// own four values.  Per four starts 17 + 4 (NWT - 7) loads instead of
+// 4 (NWT + 1).
+template <int NWT>
+__device__ __forceinline__ void window_max_default(const float* cps, int i0, int i1,
+                                                   float (&m)[kSnrMaxWidths])
+{
+    constexpr int NS = NWT < 7 ? NWT : 7;             // widths served by the block
+    constexpr int NB = kDefaultWidths[NS - 1] + 4;    // block cps[i .. i + NB)
+    constexpr int NL = NWT - NS;                      // widths with their own loads
+    int i = i0;
+    for (; i + 4 <= i1; i += 4) {
+        float blk[NB];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) blk[k] = cps[i + k];
+        float q[NL > 0 ? NL : 1][4];
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) q[j][u] = cps[i + u + kDefaultWidths[NS + j]];
+        }
+        asm volatile("" ::: "memory");
+#pragma unroll
+        for (int j = 0; j < NS; ++j) {
+            const int W = kDefaultWidths[j];
+            m[j] = fmaxf(fmaxf(m[j], blk[W] - blk[0]), blk[W + 1] - blk[1]);
+            m[j] = fmaxf(fmaxf(m[j], blk[W + 2] - blk[2]), blk[W + 3] - blk[3]);
+        }
+#pragma unroll
+        for (int j = 0; j < NL; ++j) {
+            m[NS + j] = fmaxf(fmaxf(m[NS + j], q[j][0] - blk[0]), q[j][1] - blk[1]);
+            m[NS + j] = fmaxf(fmaxf(m[NS + j], q[j][2] - blk[2]), q[j][3] - blk[3]);
+        }
+    }
+    for (; i < i1; ++i) {
+        const float c0 = cps[i];
+#pragma unroll
+        for (int j = 0; j < NWT; ++j) m[j] = fmaxf(m[j], cps[i + kDefaultWidths[j]] - c0);
+    }
+}
+
 // Per-width boxcar S/N of `nrows` finished rows staged in LDS (row p at
 // data + p*pitch), written to tk.snr.  `scratch` is free LDS.
 //  1. circular prefix sums in place (kernels.hpp:73-90): each row is cut
@@ -722,6 +767,11 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         for (int j = 0; j < kSnrMaxWidths; ++j)
             w[j] = __builtin_amdgcn_readfirstlane(s_w[j < nw ? j : nw - 1]);
         const int wmax = __builtin_amdgcn_readfirstlane(s_w[nw - 1]);
+        // default width trials (7 or more): the compile-time path
+        bool dflt = nw >= 7;
+#pragma unroll
+        for (int j = 0; j < kSnrMaxWidths; ++j)
+            if (j < nw && w[j] != kDefaultWidths[j]) dflt = false;
         __syncthreads();
         // (a) window starts [0, bins - wmax), whose windows all stay inside
         // the row, cut into the same C chunks of whole groups of 4, all
@@ -738,7 +788,16 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             float m[kSnrMaxWidths];
 #pragma unroll
             for (int j = 0; j < kSnrMaxWidths; ++j) m[j] = -INFINITY;
-            switch (nw) {
+            if (dflt) {
+                switch (nw) {
+#define RT_WIND(N) \
+    case N: window_max_default<N>(cps, i0, i1, m); break;
+                    RT_WIND(7) RT_WIND(8) RT_WIND(9) RT_WIND(10) RT_WIND(11) RT_WIND(12)
+                    RT_WIND(13) RT_WIND(14) RT_WIND(15) RT_WIND(16)
+#undef RT_WIND
+                    default: break;
+                }
+            } else switch (nw) {
 #define RT_WIN(N) \
     case N: window_max_chunk<N>(cps, i0, i1, w, m); break;
                 RT_WIN(1) RT_WIN(2) RT_WIN(3) RT_WIN(4) RT_WIN(5) RT_WIN(6) RT_WIN(7) RT_WIN(8)

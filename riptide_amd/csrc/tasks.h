@@ -69,6 +69,10 @@ constexpr int kMaxConeLevels = 7;
 // Register-resident S/N positions per lane of the standalone S/N kernel
 // (rows up to 64*kSnrRegsMaxK bins; longer rows take the chunked path).
 constexpr int kSnrRegsMaxK = 16;
+// The first kSnrMaxWidths default width trials, w <- max(w + 1, int(1.5 w))
+// from w = 1 (generate_width_trials with wtsp = 1.5).  A search whose widths
+// are a prefix of this list takes an S/N path with compile-time offsets.
+constexpr int kDefaultWidths[16] = {1, 2, 3, 4, 6, 9, 13, 19, 28, 42, 63, 94, 141, 211, 316, 474};
 // Largest number of trial widths whose results one wavefront keeps in
 // registers before flushing (one per lane).
 constexpr int kMaxWidthsPerFlush = 64;

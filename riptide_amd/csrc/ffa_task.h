@@ -584,10 +584,10 @@ __device__ __forceinline__ void window_max_chunk(const float* cps, int i0, int i
 // Window maxima of the first NWT default widths (kDefaultWidths) over the
 // starts [i0, i1): the same maxima as window_max_chunk, with compile-time
 // offsets.  The small widths (1, 2, 3, 4, 6, 9, 13) of four consecutive
-// starts read only prefix values i .. i + 16, so those are loaded once as one
-// block for the start value and all of them; the larger widths load their
-// own four values.  Per four starts 17 + 4 (NWT - 7) loads instead of
-// 4 (NWT + 1).
+// starts read only prefix values i .. i + 16: that block serves the start
+// values and all of them, and it slides by four starts per step, so only its
+// last four values are loaded; the larger widths load their own four values.
+// Per four starts 4 + 4 (NWT - 7) loads instead of 4 (NWT + 1).
 template <int NWT>
 __device__ __forceinline__ void window_max_default(const float* cps, int i0, int i1,
                                                    float (&m)[kSnrMaxWidths])
@@ -596,10 +596,17 @@ __device__ __forceinline__ void window_max_default(const float* cps, int i0, int
     constexpr int NB = kDefaultWidths[NS - 1] + 4;    // block cps[i .. i + NB)
     constexpr int NL = NWT - NS;                      // widths with their own loads
     int i = i0;
-    for (; i + 4 <= i1; i += 4) {
-        float blk[NB];
+    float blk[NB];
+    if (i + 4 <= i1) {
 #pragma unroll
-        for (int k = 0; k < NB; ++k) blk[k] = cps[i + k];
+        for (int k = 0; k < NB - 4; ++k) blk[k + 4] = cps[i + k];
+    }
+    for (; i + 4 <= i1; i += 4) {
+        // slide the block by four starts: NB - 4 values carried over
+#pragma unroll
+        for (int k = 0; k < NB - 4; ++k) blk[k] = blk[k + 4];
+#pragma unroll
+        for (int k = NB - 4; k < NB; ++k) blk[k] = cps[i + k];
         float q[NL > 0 ? NL : 1][4];
 #pragma unroll
         for (int j = 0; j < NL; ++j) {

@@ -583,18 +583,18 @@ __device__ __forceinline__ void window_max_chunk(const float* cps, int i0, int i
 
 // Window maxima of the first NWT default widths (kDefaultWidths) over the
 // starts [i0, i1): the same maxima as window_max_chunk, with compile-time
-// offsets.  The small widths (1, 2, 3, 4, 6, 9, 13) of four consecutive
-// starts read only prefix values i .. i + 16: that block serves the start
+// offsets.  The small widths (1, 2, 3, 4, 6, 9, 13, 19) of four consecutive
+// starts read only prefix values i .. i + 22: that block serves the start
 // values and all of them, and it slides by four starts per step, so only its
 // last four values are loaded; the larger widths load their own four values.
-// Per four starts 4 + 4 (NWT - 7) loads instead of 4 (NWT + 1).
+// Per four starts 4 + 4 (NWT - 8) loads instead of 4 (NWT + 1).
 template <int NWT>
 __device__ __forceinline__ void window_max_default(const float* cps, int i0, int i1,
                                                    float (&m)[kSnrMaxWidths])
 {
-    constexpr int NS = NWT < 7 ? NWT : 7;             // widths served by the block
-    constexpr int NB = kDefaultWidths[NS - 1] + 4;    // block cps[i .. i + NB)
-    constexpr int NL = NWT - NS;                      // widths with their own loads
+    constexpr int NS = NWT < 8 ? NWT : 8;  // widths served by the block
+    constexpr int NB = kDefaultWidths[NS - 1] + 4;  // block cps[i .. i + NB)
+    constexpr int NL = NWT - NS;  // widths with their own loads
     int i = i0;
     float blk[NB];
     if (i + 4 <= i1) {

@@ -485,6 +485,10 @@ struct PhaseTimer {
 
 static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
 {
+    // every run starts with a clear error flag (a fill kernel, not a memset
+    // node, so that the run can be captured into a graph)
+    hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, b->d_err, 1, 0);
+    BT_HIP(hipGetLastError());
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         const Group& g = b->groups[gi];
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 0], s));

@@ -115,12 +115,14 @@ def _snr(x, widths, stdnoise):
     rows, cols = x.shape
     stdnoise = float(np.float32(stdnoise))
     out_shape = (rows, w.size)
+    # argument checks in the reference's order (python_bindings.cpp:131-132:
+    # check_stdnoise, then check_trial_widths, snr.hpp:14-32), on the host
+    # before any device work; rt_snr2 repeats them
+    if not stdnoise > 0:
+        raise ValueError("stdnoise must be > 0")
+    if not bool(np.all((w > 0) & (w < cols))):
+        raise ValueError("trial widths must be all > 0 and < columns")
     if rows == 0 or w.size == 0:
-        # argument checks still apply
-        if not stdnoise > 0:
-            raise ValueError("stdnoise must be > 0")
-        if not all((wi > 0) and (wi < cols) for wi in w.tolist()):
-            raise ValueError("trial widths must be all > 0 and < columns")
         return np.zeros(out_shape, dtype=np.float32)
     d = _lib.to_device(x)
     out = _lib.empty(out_shape)

@@ -26,8 +26,11 @@ def ffa1(data, p):
     N % p samples are ignored (libffa.py:94-126)."""
     if not data.ndim == 1:
         raise ValueError("input data must be one-dimensional")
-    if not (isinstance(p, int) and p > 0):
+    # numpy integers are accepted too (the reference's isinstance(p, int)
+    # rejects them, libffa.py:121)
+    if isinstance(p, (bool, np.bool_)) or not (isinstance(p, (int, np.integer)) and p > 0):
         raise ValueError("p must be an integer > 1")
+    p = int(p)
     if p > data.size:
         raise ValueError("p must be smaller than the total number of samples")
     m = data.size // p

@@ -157,9 +157,17 @@ class PeakFinder:
             polycos[iw] = polyco
             coefs.append(np.asarray(polyco, np.float64))
         ncoef = max(len(c) for c in coefs)
-        C = np.zeros((nw, ncoef), np.float32)
-        for iw, c in enumerate(coefs):
-            C[iw, ncoef - len(c):] = c  # leading zeros keep the degree
+        # thresholds of degree > 8 are beyond the device evaluation: the
+        # device pass then applies smin only (a threshold far below every
+        # S/N) and the host decides every selected value in float64
+        host_threshold = ncoef > MAX_DEVICE_COEFS
+        if host_threshold:
+            ncoef = 1
+            C = np.full((nw, 1), -3.0e38, np.float32)
+        else:
+            C = np.zeros((nw, ncoef), np.float32)
+            for iw, c in enumerate(coefs):
+                C[iw, ncoef - len(c):] = c  # leading zeros keep the degree
         dev = snrs.device
         logf32 = self._device_arrays()
         d_coef = torch.tensor(C, device=dev)
@@ -177,6 +185,9 @@ class PeakFinder:
             ns, nu = int(cnt[iw, 0]), int(cnt[iw, 1])
             sure = sel[iw, :ns].cpu().numpy().astype(np.int64)
             maybe = unc[iw, :nu].cpu().numpy().astype(np.int64)
+            if host_threshold:
+                sure, maybe = sure[:0], np.concatenate([sure, maybe])
+                nu = maybe.size
             if nu:
                 s_maybe = snrs[torch.as_tensor(maybe, device=dev), iw].cpu().numpy().astype(np.float64)
                 thr = np.array([_horner(polycos[iw], self.logf[i]) for i in maybe])
@@ -194,6 +205,11 @@ class PeakFinder:
                                   iw=int(iw), ip=ipeak, snr=snr, dm=dm))
         peaks = sorted(peaks, key=lambda pk: pk.snr, reverse=True)
         return peaks, polycos
+
+
+# Coefficients of the largest threshold polynomial the device pass evaluates
+# (degree 8, rt_peak_candidates); larger degrees are decided on the host.
+MAX_DEVICE_COEFS = 9
 
 
 def find_peaks(pgram, smin=6.0, segwidth=5.0, nstd=6.0, minseg=10, polydeg=2, clrad=0.1):

@@ -76,3 +76,20 @@ def test_find_peaks_on_gpu_periodogram():
     assert abs(got[0].period / p0 - 1) < 2e-3
     peaks, polycos = find_peaks(pg)
     assert [tuple(p) for p in peaks] == [tuple(p) for p in got]
+
+
+def test_find_peaks_high_degree_threshold():
+    """Threshold polynomials of degree 9 and 11 (beyond the device
+    evaluation, which takes up to degree 8): the device pass applies smin
+    only and the host decides the threshold in float64; peaks and
+    polynomials are the oracle's (np.polyfit accepts any degree,
+    peak_detection.py:111-115)."""
+    rng = np.random.default_rng(15)
+    L, nw = 120000, 3
+    periods = np.sort(rng.uniform(0.3, 3.0, L))[::-1].copy()
+    snrs = (rng.standard_normal((L, nw)) * 1.2 + 4.0).astype(np.float32)
+    for ip in (500, 60000, 119000):
+        snrs[ip - 3:ip + 3, :] += 11.0
+    foldbins = rng.integers(240, 261, L).astype(np.uint32)
+    for deg in (8, 9, 11):
+        _compare(periods, foldbins, np.array([1, 3, 7]), snrs, 800.0, polydeg=deg)

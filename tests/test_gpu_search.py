@@ -82,6 +82,40 @@ def test_deredden_parity(oracle, n, tsamp, width, minpts):
     assert np.mean(got == want) > 0.99
 
 
+@pytest.mark.parametrize("sf,extra", [(4, 0), (5, 3), (9, 8), (154, 77), (618, 1)])
+def test_deredden_ramp_exact(oracle, sf, extra):
+    """A ramp dereddens to exactly zero between the first and last block
+    centre (i*sf + (sf-1)/2) and to the ramp minus the end blocks' medians
+    outside, also with a trailing partial block (N % sf != 0): every value
+    is an integer or half-integer, so the GPU result is exact."""
+    from riptide_amd.time_series import deredden_array
+    minpts, nl = 11, 40
+    n = nl * sf + extra
+    x = (2.0 * np.arange(n) - 300.0).astype(np.float32)
+    width = sf * minpts + minpts // 2
+    got = deredden_array(x, width, minpts)
+    want = oracle.deredden(x, 1.0, float(width), minpts)
+    assert np.array_equal(got, want)
+    c0, c1 = (sf - 1) // 2, (nl - 1) * sf + (sf - 1) // 2
+    assert np.all(got[c0 + 1:c1] == 0.0)
+
+
+@pytest.mark.parametrize("n,sf,minpts", [(5003, 7, 11), (20011, 154, 51), (1_000_003, 618, 101)])
+def test_deredden_trailing_block_parity(oracle, n, sf, minpts):
+    """Red noise with N % sf != 0: the trailing partial block is dropped
+    from the block means and the values past the last centre take the last
+    block's median."""
+    from riptide_amd.time_series import deredden_array
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 5 * np.sin(np.arange(n) / (3.0 * sf))).astype(np.float32)
+    width = sf * minpts + 1
+    got = deredden_array(x, width, minpts)
+    want = oracle.deredden(x, 1.0, float(width), minpts)
+    scale = np.abs(x).max()
+    np.testing.assert_allclose(got, want, rtol=0, atol=2 * np.spacing(np.float32(scale)))
+    assert np.mean(got == want) > 0.99
+
+
 def test_normalise_parity(oracle):
     from riptide_amd.time_series import normalise_array
     rng = np.random.default_rng(11)
@@ -157,6 +191,44 @@ def test_periodogram_parity_shapes(oracle, case, shape):
     N, tsamp, pmin, pmax, bmin, bmax, widths = case
     x = oracle.normalise(_pulsar(N, tsamp, 0.37 * (pmin + pmax), 0.05, 15.0, seed=bmin + shape))
     w = oracle.generate_width_trials(bmin) if widths is None else np.asarray(widths)
+    tune.set_shape(shape)
+    try:
+        P, F, S = libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    finally:
+        tune.set_shape(None)
+    Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    assert np.array_equal(P, Po) and np.array_equal(F, Fo)
+    np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)
+
+
+# The S/N phase's compile-time path for the default width trials holds one
+# instantiation per width count (7-16): the sliding block serves the first
+# min(N, 8) widths and the rest load their own values.  These bins ranges give
+# N = 7, 8, 9, 11, 14, 15 and 16 default widths (240, 480 and 800 give 10, 12
+# and 13 above); every workgroup shape.
+DEFAULT_WIDTH_CASES = [
+    # bins_min, samples -> number of default widths
+    (70, 2 ** 17),     # 7: fewer widths than the block serves
+    (100, 2 ** 17),    # 8: exactly the block, no other widths
+    (150, 2 ** 17),    # 9
+    (320, 2 ** 17),    # 11
+    (1100, 2 ** 18),   # 14
+    (1600, 2 ** 18),   # 15
+    (2400, 2 ** 18),   # 16: all of kDefaultWidths
+]
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("bmin,N", DEFAULT_WIDTH_CASES)
+def test_periodogram_default_width_counts(oracle, bmin, N, shape):
+    from riptide_amd import libcpp, tune
+    tsamp = 1e-3
+    pmin = bmin * tsamp * 1.05
+    pmax = 1.6 * pmin
+    bmax = bmin + max(4, bmin // 20)
+    w = oracle.generate_width_trials(bmin)
+    assert w.size == {70: 7, 100: 8, 150: 9, 320: 11, 1100: 14, 1600: 15, 2400: 16}[bmin]
+    x = oracle.normalise(_pulsar(N, tsamp, 1.3 * pmin, 0.04, 15.0, seed=bmin + 10 * shape))
     tune.set_shape(shape)
     try:
         P, F, S = libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)

@@ -323,6 +323,128 @@ def test_deredden_constant_is_zero(oracle):
     assert np.allclose(oracle.deredden(x, 1e-3, 0.5, 51), 0.0, atol=1e-6)
 
 
+# ---------------------------------------------------------------------------
+# Dereddening with scrunching (running_medians.py:40-85, time_series.py:95-124)
+# ---------------------------------------------------------------------------
+
+def numpy_pairwise_sum_f32(a):
+    """numpy's pairwise float32 summation (pairwise_sum in numpy's
+    umath/loops_utils.h.src, the inner loop of add.reduce over a contiguous
+    axis, started from the additive identity): blocks of at most 128 values
+    summed with eight interleaved accumulators combined as
+    ((r0+r1)+(r2+r3))+((r4+r5)+(r6+r7)), then the rest one by one; longer
+    runs split at a multiple of 8 near the middle."""
+    n = a.size
+    f = np.float32
+    if n < 8:
+        r = f(0.0)
+        for v in a:
+            r = f(r + v)
+        return r
+    if n <= 128:
+        r = [f(v) for v in a[:8]]
+        i = 8
+        while i < n - n % 8:
+            for j in range(8):
+                r[j] = f(r[j] + a[i + j])
+            i += 8
+        res = f(f(f(r[0] + r[1]) + f(r[2] + r[3])) + f(f(r[4] + r[5]) + f(r[6] + r[7])))
+        while i < n:
+            res = f(res + a[i])
+            i += 1
+        return res
+    n2 = n // 2
+    n2 -= n2 % 8
+    return f(numpy_pairwise_sum_f32(a[:n2]) + numpy_pairwise_sum_f32(a[n2:]))
+
+
+@pytest.mark.parametrize("sf", [2, 3, 8, 9, 100, 128, 129, 154, 309, 618])
+def test_scrunch_is_numpy_pairwise_mean(oracle, sf):
+    """The block means are numpy's float32 pairwise sums divided by the
+    float32 factor, bit for bit, including the trailing partial block being
+    dropped (N % sf != 0)."""
+    rng = np.random.default_rng(sf)
+    x = (rng.standard_normal(sf * 11 + sf // 2 + 1) * 100 + 7).astype(np.float32)
+    got = oracle.scrunch(x, sf)
+    assert got.dtype == np.float32 and got.size == x.size // sf
+    want = np.array([numpy_pairwise_sum_f32(x[j * sf:(j + 1) * sf]) / np.float32(sf)
+                     for j in range(x.size // sf)], dtype=np.float32)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("sf,extra", [(4, 0), (5, 3), (9, 8), (154, 77), (618, 1)])
+def test_fast_running_median_reproduces_a_ramp(oracle, sf, extra):
+    """On a ramp x[i] = a*i + b the block means are the ramp at the block
+    centres i*sf + (sf-1)/2, the running median of a monotonic sequence is the
+    sequence itself (edge padding included), and linear interpolation over the
+    centres reproduces the ramp between the first and last centre and clamps
+    to the end values outside.  A centre off by half a sample (e.g. i*sf +
+    sf/2) moves every interior value by a/2 and fails this test; the values
+    are integers and half-integers, so every step is exact."""
+    minpts = 11
+    nl = 40
+    n = nl * sf + extra  # the trailing `extra` samples form no block
+    a, b = 2.0, -300.0
+    x = (a * np.arange(n) + b).astype(np.float32)
+    width = sf * minpts + minpts // 2  # int(width / minpts) == sf
+    got = oracle.fast_running_median(x, width, minpts)
+    assert got.dtype == np.float64 and got.size == n
+    c0, c1 = 0.5 * (sf - 1), (nl - 1) * sf + 0.5 * (sf - 1)
+    i = np.arange(n, dtype=np.float64)
+    want = a * np.clip(i, c0, c1) + b
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-9 * abs(b))
+    # the trailing samples past the last centre hold the last block's median
+    assert np.all(got[int(np.ceil(c1)):] == got[-1])
+
+
+@pytest.mark.parametrize("n,sf,minpts", [(5003, 7, 11), (20011, 154, 51), (100_000, 618, 101)])
+def test_fast_running_median_interpolation_restated(oracle, n, sf, minpts):
+    """Element-by-element check of the interpolation step against an explicit
+    float64 linear interpolation over the centres c_j = j*sf + (sf-1)/2
+    (index j found by integer arithmetic, not by search), clamped to the end
+    values outside [c_0, c_last], on noisy red data with N % sf != 0."""
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) + 5 * np.sin(np.arange(n) / (3.0 * sf))).astype(np.float32)
+    width = sf * minpts + 1
+    got = oracle.fast_running_median(x, width, minpts)
+    lo = oracle.oracle().running_median(oracle.scrunch(x, sf), minpts).astype(np.float64)
+    nl = lo.size
+    assert nl == n // sf
+    want = np.empty(n)
+    for i in range(n):
+        # 2*c_j = 2*j*sf + sf - 1: the last centre at or below i
+        j = (2 * i - (sf - 1)) // (2 * sf)
+        if j < 0:
+            want[i] = lo[0]
+        elif j >= nl - 1:
+            want[i] = lo[-1]
+        else:
+            t = (i - (j * sf + 0.5 * (sf - 1))) / sf
+            want[i] = lo[j] + t * (lo[j + 1] - lo[j])
+    np.testing.assert_allclose(got, want, rtol=1e-14, atol=1e-14 * np.abs(lo).max())
+    # at the centres themselves (integers when sf is odd) the value is the
+    # low-resolution median exactly
+    if sf % 2:
+        centres = np.arange(nl) * sf + (sf - 1) // 2
+        assert np.array_equal(got[centres], lo)
+
+
+def test_deredden_rounds_float64_difference(oracle):
+    """time_series.py:121-124 then TimeSeries(...) (time_series.py:41-45):
+    float32 data minus the float64 running median, rounded once to float32."""
+    rng = np.random.default_rng(77)
+    n, tsamp = 30_011, 1e-3
+    x = (rng.standard_normal(n) * 3 + np.linspace(0, 50, n)).astype(np.float32)
+    got = oracle.deredden(x, tsamp, 1.0, 51)
+    rmed = oracle.fast_running_median(x, 1000, 51)
+    assert got.dtype == np.float32
+    assert np.array_equal(got, (x.astype(np.float64) - rmed).astype(np.float32))
+    # a linear trend is removed away from the ends (first/last half block)
+    sf = int(1000 / 51)
+    mid = slice(sf, n - 2 * sf)
+    assert abs(np.mean(got[mid])) < 0.1
+
+
 def test_normalise_moments(oracle):
     x = (np.random.default_rng(0).standard_normal(100_000) * 5 + 3).astype(np.float32)
     y = oracle.normalise(x)

@@ -91,6 +91,21 @@ int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const ui
                          int64_t bins_max, double* d_periods, uint32_t* d_foldbins,
                          float* d_snrs, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rt_periodogram with every intermediate buffer carved from
+ * a static device array of 256 MiB per device instead of the library's
+ * scratch pool; no hipMalloc / hipFree.  Identical results.  Searches whose
+ * intermediates exceed the array (roughly: packed downsampled series, the
+ * largest transform's two intermediate states and 24 MiB of task list) are
+ * rejected with RT_EINVAL.  Calls are serialised by the library's lock and
+ * wait for their work; the array keeps the last call's contents, and calls
+ * that overlap on one GPU from several processes each use their own copy. */
+int rt_periodogram_noalloc(const float* d_in, int64_t size, double tsamp,
+                           const uint64_t* h_widths, int64_t nw, double period_min,
+                           double period_max, int64_t bins_min, int64_t bins_max,
+                           double* d_periods, uint32_t* d_foldbins, float* d_snrs, void* stream);
+
 /* libcpp.running_median (python_bindings.cpp:212-222 -> running_median.hpp:100-132). */
 int rt_running_median(const float* d_in, int64_t size, int64_t width, float* d_out,
                       void* stream);

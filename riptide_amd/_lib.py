@@ -34,6 +34,8 @@ _SIGNATURES = {
                        _c_ptr, _c_ptr, _c_ptr, _c_ptr],
     "rt_periodogram_plain": [_c_ptr, _c_i64, _c_dbl, _c_ptr, _c_i64, _c_dbl, _c_dbl, _c_i64,
                              _c_i64, _c_ptr, _c_ptr, _c_ptr, _c_ptr],
+    "rt_periodogram_noalloc": [_c_ptr, _c_i64, _c_dbl, _c_ptr, _c_i64, _c_dbl, _c_dbl, _c_i64,
+                               _c_i64, _c_ptr, _c_ptr, _c_ptr, _c_ptr],
     "rt_running_median": [_c_ptr, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_rollback": [_c_ptr, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_fused_rollback_add": [_c_ptr, _c_ptr, _c_i64, _c_i64, _c_ptr, _c_ptr],

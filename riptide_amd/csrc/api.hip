@@ -87,11 +87,38 @@ static int pool_get(int slot, size_t bytes, void** out)
     return RT_OK;
 }
 
+// Regions carved in order, 256-byte aligned, from one device buffer that the
+// caller owns (the allocation-free entry point's static array): the
+// alternative to the pool for calls that must not allocate.
+struct Arena {
+    char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+};
+
+static int arena_get(Arena* a, size_t bytes, void** out)
+{
+    const size_t off = (a->used + 255) & ~(size_t)255;
+    if (off > a->cap || bytes > a->cap - off)
+        return fail(RT_EINVAL, "periodogram needs more scratch than the allocation-free entry "
+                               "point holds (" + std::to_string(a->cap >> 20) + " MiB)");
+    *out = a->base + off;
+    a->used = off + bytes;
+    return RT_OK;
+}
+
+// A buffer of a pool slot, or the next region of `arena` when one is given.
+static int scratch_get(Arena* arena, int slot, size_t bytes, void** out)
+{
+    return arena ? arena_get(arena, bytes, out) : pool_get(slot, bytes, out);
+}
+
 template <typename T>
-static int upload(int slot, const std::vector<T>& v, T** d, hipStream_t stream)
+static int upload(int slot, const std::vector<T>& v, T** d, hipStream_t stream,
+                  Arena* arena = nullptr)
 {
     void* p = nullptr;
-    int st = pool_get(slot, v.size() * sizeof(T), &p);
+    int st = scratch_get(arena, slot, v.size() * sizeof(T), &p);
     if (st) return st;
     if (!v.empty())
         RT_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, stream));
@@ -140,7 +167,7 @@ static std::vector<float> hb_table(int64_t b0, int64_t nb, const uint64_t* w, in
 // Uploads all passes' tasks, launches them in order, waits and checks the
 // kernel's overflow flag.
 static int run_passes(const std::vector<std::vector<FfaTask>>& passes, hipStream_t stream,
-                      bool sync, int shape)
+                      bool sync, int shape, Arena* arena = nullptr)
 {
     // the tasks of a pass are independent: grouped by kernel variant, one
     // launch per variant
@@ -162,10 +189,10 @@ static int run_passes(const std::vector<std::vector<FfaTask>>& passes, hipStream
     }
     if (all.empty()) return RT_OK;
     FfaTask* d_tasks = nullptr;
-    int st = upload(kSlotTasks, all, &d_tasks, stream);
+    int st = upload(kSlotTasks, all, &d_tasks, stream, arena);
     if (st) return st;
     void* errp = nullptr;
-    st = pool_get(kSlotErr, sizeof(int), &errp);
+    st = scratch_get(arena, kSlotErr, sizeof(int), &errp);
     if (st) return st;
     RT_HIP(hipMemsetAsync(errp, 0, sizeof(int), stream));
     for (size_t i = 0; i < passes.size(); ++i)
@@ -638,7 +665,8 @@ struct PgramSetup {
 // tables and runs the packed downsampling of every step.
 static int pgram_setup(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
                        int64_t nw, double pmin, double pmax, int64_t bmin, int64_t bmax,
-                       double* d_periods, uint32_t* d_foldbins, hipStream_t s, PgramSetup* out)
+                       double* d_periods, uint32_t* d_foldbins, hipStream_t s, PgramSetup* out,
+                       Arena* arena = nullptr)
 {
     const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &out->plan);
     if (!e.empty()) return fail(RT_EINVAL, e);
@@ -665,14 +693,15 @@ static int pgram_setup(const float* d_in, int64_t size, double tsamp, const uint
         std::memcpy(&tab[woff + (size_t)i], &w, sizeof(int32_t));
     }
     float* d_tab = nullptr;
-    st = upload(kSlotTables, tab, &d_tab, s);
+    st = upload(kSlotTables, tab, &d_tab, s, arena);
     if (st) return st;
     out->d_hb = d_tab;
     out->d_w = (int32_t*)(d_tab + woff);
 
     // packed downsampling of every f > 1 step
     void* dsp = nullptr;
-    st = pool_get(kSlotDs, (size_t)std::max<int64_t>(p.ds_total, 1) * sizeof(float), &dsp);
+    st = scratch_get(arena, kSlotDs, (size_t)std::max<int64_t>(p.ds_total, 1) * sizeof(float),
+                     &dsp);
     if (st) return st;
     out->d_ds = (float*)dsp;
     std::vector<DsStepDev> steps;
@@ -710,7 +739,7 @@ static int pgram_setup(const float* d_in, int64_t size, double tsamp, const uint
         std::memcpy(blob.data(), &sd, sizeof(sd));
         std::memcpy(blob.data() + sizeof(sd), steps.data(), steps.size() * sizeof(DsStepDev));
         uint8_t* d_blob = nullptr;
-        st = upload(kSlotMisc, blob, &d_blob, s);
+        st = upload(kSlotMisc, blob, &d_blob, s, arena);
         if (st) return st;
         RT_HIP(launch_downsample_packed((const DsSeriesDev*)d_blob, 1,
                                         (const DsStepDev*)(d_blob + sizeof(DsSeriesDev)), nchunks,
@@ -719,6 +748,27 @@ static int pgram_setup(const float* d_in, int64_t size, double tsamp, const uint
     // the host staging vectors above die with this frame
     RT_HIP(hipStreamSynchronize(s));
     return RT_OK;
+}
+
+// Task specification of transform `t` of step `stp` (S/N into d_snrs); the
+// intermediate buffers are left to the caller.
+static TransformSpec pgram_spec(const PgramSetup& su, const float* d_in, const Step& stp,
+                                const Transform& t, float* d_snrs)
+{
+    const SeriesPlan& p = su.plan;
+    TransformSpec sp;
+    sp.m = t.rows;
+    sp.bins = t.bins;
+    sp.cap = ffa_cap_rows(t.bins, su.shape);
+    sp.src = (stp.ds_offset < 0) ? d_in : su.d_ds + stp.ds_offset;
+    sp.kind = 1;
+    sp.snr = d_snrs + t.out_row * su.nw;
+    sp.widths = su.d_w;
+    sp.hb = su.d_hb + (size_t)(t.bins - p.bmin) * su.nw * 2;
+    sp.nw = (int)su.nw;
+    sp.stdnoise = t.stdnoise;
+    sp.rows_eval = t.rows_eval;
+    return sp;
 }
 
 // Builds the tasks of every transform of steps [s0, s1) into `passes`, with
@@ -730,25 +780,12 @@ static int64_t pgram_tasks(const PgramSetup& su, const float* d_in, int s0, int 
     int64_t used = 0;
     for (int is = s0; is < s1; ++is) {
         const Step& stp = p.steps[is];
-        const float* src = (stp.ds_offset < 0) ? d_in : su.d_ds + stp.ds_offset;
         for (int it = stp.first_tr; it < stp.first_tr + stp.ntr; ++it) {
             const Transform& t = p.tr[it];
             if (t.rows_eval <= 0) continue;
-            const int cap = ffa_cap_rows(t.bins, su.shape);
             const int64_t n = (int64_t)t.rows * t.bins;
-            TransformSpec sp;
-            sp.m = t.rows;
-            sp.bins = t.bins;
-            sp.cap = cap;
-            sp.src = src;
-            sp.kind = 1;
-            sp.snr = d_snrs + t.out_row * su.nw;
-            sp.widths = su.d_w;
-            sp.hb = su.d_hb + (size_t)(t.bins - p.bmin) * su.nw * 2;
-            sp.nw = (int)su.nw;
-            sp.stdnoise = t.stdnoise;
-            sp.rows_eval = t.rows_eval;
-            if (t.rows > cap) {
+            TransformSpec sp = pgram_spec(su, d_in, stp, t, d_snrs);
+            if (t.rows > sp.cap) {
                 if (!dry) {
                     sp.buf_a = inter + used;
                     sp.buf_b = inter + used + n;
@@ -825,6 +862,129 @@ int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const ui
     for (auto& ps : per_step)
         for (auto& p : ps) flat.push_back(p);
     return run_passes(flat, s, true, su.shape);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free periodogram
+//
+// `scratch` is the only device memory rt_periodogram_noalloc uses besides the
+// caller's arrays: the S/N tables, the downsampling descriptors, the packed
+// downsampled series, the FFA intermediate states, the task list and the
+// error flag are carved from it on every call.  The array's name is a
+// host-side stand-in for it: kernels and copies get the device address that
+// hipGetSymbolAddress returns, looked up once per device.  Every device the
+// library runs on holds its own copy until the process ends.
+// ---------------------------------------------------------------------------
+
+namespace rt {
+
+constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
+// part of it kept for the task list (about 280k tasks)
+constexpr size_t kNoAllocTaskBytes = (size_t)24 << 20;
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::map<int, char*> g_scratch_dev;  // device -> address of `scratch`
+
+static int scratch_address(char** out)
+{
+    int dev = 0;
+    RT_HIP(hipGetDevice(&dev));
+    auto it = g_scratch_dev.find(dev);
+    if (it == g_scratch_dev.end()) {
+        void* p = nullptr;
+        RT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (p == nullptr) return fail(RT_EDEVICE, "no device address for the scratch array");
+        it = g_scratch_dev.emplace(dev, (char*)p).first;
+    }
+    *out = it->second;
+    return RT_OK;
+}
+
+}  // namespace rt
+
+extern "C" {
+
+int rt_periodogram_noalloc(const float* d_in, int64_t size, double tsamp,
+                           const uint64_t* h_widths, int64_t nw, double pmin, double pmax,
+                           int64_t bmin, int64_t bmax, double* d_periods, uint32_t* d_foldbins,
+                           float* d_snrs, void* stream)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    hipStream_t s = S(stream);
+    // sizes first, on the host: nothing runs unless everything fits
+    {
+        SeriesPlan p;
+        const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &p);
+        if (!e.empty()) return fail(RT_EINVAL, e);
+        int st = check_widths(h_widths, nw, bmin);
+        if (st) return st;
+        const int shape = current_ffa_shape();
+        int64_t big = 0;  // largest intermediate need of one transform (floats)
+        for (const Transform& t : p.tr)
+            if (t.rows_eval > 0 && t.rows > ffa_cap_rows(t.bins, shape))
+                big = std::max<int64_t>(big, 2 * (int64_t)t.rows * t.bins);
+        const size_t fixed = 4096 + (size_t)((bmax - bmin + 1) * nw * 2 + nw) * 4 +
+                             sizeof(DsSeriesDev) + p.steps.size() * sizeof(DsStepDev) +
+                             (size_t)std::max<int64_t>(p.ds_total, 1) * 4 + kNoAllocTaskBytes;
+        if (fixed + (size_t)big * 4 > kNoAllocScratchBytes)
+            return fail(RT_EINVAL,
+                        "periodogram needs " + std::to_string((fixed + big * 4) >> 20) +
+                            " MiB of scratch, more than the " +
+                            std::to_string(kNoAllocScratchBytes >> 20) +
+                            " MiB the allocation-free entry point holds");
+    }
+    char* base = nullptr;
+    int st = scratch_address(&base);
+    if (st) return st;
+    // [tables | downsampling descriptors | downsampled series | intermediates][task list]
+    Arena a{base, kNoAllocScratchBytes - kNoAllocTaskBytes, 0};
+    Arena ta{base + (kNoAllocScratchBytes - kNoAllocTaskBytes), kNoAllocTaskBytes, 0};
+    PgramSetup su;
+    st = pgram_setup(d_in, size, tsamp, h_widths, nw, pmin, pmax, bmin, bmax, d_periods,
+                     d_foldbins, s, &su, &a);
+    if (st) return st;
+    if (su.plan.length == 0 || nw == 0) {
+        RT_HIP(hipStreamSynchronize(s));
+        return RT_OK;
+    }
+    // the rest holds the intermediate states of a chunk of transforms; the
+    // chunks run one after the other on the stream, each reusing it
+    const size_t off = (a.used + 255) & ~(size_t)255;
+    const int64_t inter_floats = (int64_t)((a.cap - std::min(a.cap, off)) / 4);
+    void* ip = nullptr;
+    st = arena_get(&a, (size_t)inter_floats * 4, &ip);
+    if (st) return st;
+    float* inter = (float*)ip;
+    std::vector<std::vector<FfaTask>> all, chunk;
+    int64_t used = 0;
+    auto flush = [&]() {
+        for (auto& q : chunk) all.push_back(std::move(q));
+        chunk.clear();
+        used = 0;
+    };
+    const SeriesPlan& p = su.plan;
+    for (const Step& stp : p.steps)
+        for (int it = stp.first_tr; it < stp.first_tr + stp.ntr; ++it) {
+            const Transform& t = p.tr[it];
+            if (t.rows_eval <= 0) continue;
+            TransformSpec sp = pgram_spec(su, d_in, stp, t, d_snrs);
+            if (t.rows > sp.cap) {
+                const int64_t n = (int64_t)t.rows * t.bins;
+                if (2 * n > inter_floats)
+                    return fail(RT_EINVAL, "FFA intermediate states exceed the scratch array");
+                if (used + 2 * n > inter_floats) flush();
+                sp.buf_a = inter + used;
+                sp.buf_b = inter + used + n;
+                used += 2 * n;
+            }
+            if (plan_transform_tasks(sp, chunk) < 0)
+                return fail(RT_EINVAL, "bins too large for the FFA tiling");
+        }
+    flush();
+    return run_passes(all, s, true, su.shape, &ta);
 }
 
 }  // extern "C"

@@ -27,6 +27,11 @@ if ROOT not in sys.path:
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 LDS_PEAK_GBS = 78_600.0        # 128 B/clk/CU x 256 CU x 2.4 GHz (nominal)
+# ds_read_b32 rate measured on gfx950 (profiles/lds_read_rates__gfx950.md):
+# 169 B/clk/CU x 256 CU x 2.4 GHz
+LDS_MEASURED_GBS = 169.0 * 256 * 2.4
+# host cores a one-GPU box gives a job (its CPU share), whatever the machine has
+HOST_CORES_PER_GPU = 16
 
 
 def log(msg):
@@ -81,24 +86,128 @@ def _cpu_worker(args):
     return times
 
 
+def host_cpu():
+    """Host CPU model, logical CPU count and the CPUs this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"model": model, "cpu_count": os.cpu_count(), "affinity": allowed}
+
+
 def cpu_baseline(nsamp, tsamp):
+    """The oracle's ffa_search (C restatement of the reference kernels, as fast
+    as the compiled reference: ratio 1.00-1.06 on 2^22 trials) in one
+    single-threaded process per host core, as the reference's DM-trial pool
+    runs (pipeline/worker_pool.py:35-45)."""
     import multiprocessing as mp
-    cores = max(1, min(16, os.cpu_count() or 1))
+    info = host_cpu()
+    cores = max(1, min(HOST_CORES_PER_GPU, info["affinity"]))
     ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
     with ctx.Pool(cores) as pool:
         res = pool.map(_cpu_worker, [(nsamp, tsamp, 100 + c) for c in range(cores)])
-    wall = max(sum(r) for r in res)
+    wall = time.perf_counter() - t0
+    busy = max(sum(r) for r in res)
     per = np.mean(np.asarray(res), axis=0)
     return {
-        "value": round(2 * cores / wall, 4),
+        "value": round(2 * cores / busy, 4),
         "unit": "DM trials/s",
         "cores": cores,
         "kind": "port",
         "sample": (f"oracle ffa_search (C restatement of the reference kernels + its numpy "
                    f"dereddening/normalisation), one 2^22-sample trial of each bins class per "
-                   f"core, {cores} single-threaded processes; per-core s/trial: "
-                   f"bins 240-260 {per[0]:.2f}, bins 480-520 {per[1]:.2f}"),
+                   f"core, {cores} single-threaded processes ({wall:.1f} s wall); per-core "
+                   f"s/trial: bins 240-260 {per[0]:.2f}, bins 480-520 {per[1]:.2f}"),
+        "host_cpu": info["model"],
+        "host_cpu_count": info["cpu_count"],
+        "cores_note": (f"{cores} processes: the {HOST_CORES_PER_GPU} host cores a one-GPU box "
+                       f"gives a job (the machine reports {info['cpu_count']} logical CPUs, "
+                       f"{info['affinity']} in this process's affinity)"),
     }
+
+
+def snr_errors(got, want):
+    """Largest absolute and relative S/N differences (relative over |want| >= 1)."""
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    d = np.abs(got - want)
+    big = np.abs(want) >= 1.0
+    return {"max_abs": float(d.max()),
+            "max_rel": float((d[big] / np.abs(want[big])).max()) if big.any() else 0.0}
+
+
+def oracle_checks(batch, xs, outs, coin, bmin, bmax, tsamp, pmin, pmax, c1, c2, peaks_ctx):
+    """Checks of the GPU results made with the oracle on the host, in the
+    CPU-baseline leg: parity of the first batch series of each bins class,
+    the S/N error of GPU and oracle against a float64 recomputation on a
+    config-1 series, the config-2 search (oracle seconds and parity) and the
+    host path of find_peaks (S/N copy + the oracle's numpy find_peaks)."""
+    from oracle import oracle as O
+    from oracle import peaks as OP
+    out = {}
+    parity = []
+    for cls in (0, 1):
+        i = int(np.nonzero(coin == cls)[0][0])
+        x0 = xs[i].cpu().numpy()
+        t0 = time.perf_counter()
+        _, _, Po, Fo, So = O.ffa_search(x0, tsamp, period_min=pmin, period_max=pmax,
+                                        bins_min=int(bmin[i]), bins_max=int(bmax[i]))
+        t_or = time.perf_counter() - t0
+        S0 = outs[i].cpu().numpy()
+        P0, F0 = batch.grid(i)
+        parity.append({
+            "series": i, "bins": f"{int(bmin[i])}-{int(bmax[i])}",
+            "periods_bit_exact": bool(np.array_equal(P0, Po) and np.array_equal(F0, Fo)),
+            "snr_max_abs_diff": float(np.max(np.abs(S0 - So))),
+            "snr_within_1e-5rel_1e-4abs": bool(np.allclose(S0, So, rtol=1e-5, atol=1e-4)),
+            "snr_bit_identical_frac": round(float(np.mean(S0 == So)), 6),
+            "oracle_s": round(t_or, 2)})
+    out["parity"] = parity
+    # float64 recomputation of a config-1 series (the oracle's C code in float64)
+    x1, tsamp1, kw1, S_gpu = c1
+    xn = O.normalise(O.deredden(x1, tsamp1, 4.0))
+    w1 = O.generate_width_trials(kw1["bins_min"])
+    args = (kw1["period_min"], kw1["period_max"], kw1["bins_min"], kw1["bins_max"])
+    _, _, S32 = O.oracle().periodogram(xn, tsamp1, w1, *args)
+    S64 = O.oracle().periodogram_f64(xn, tsamp1, w1, *args)
+    out["snr_vs_fp64"] = {
+        "config": "2^20 samples, 256 us, P 0.5-2 s, bins 240-260 (config 1)",
+        "gpu": snr_errors(S_gpu, S64), "oracle_float32": snr_errors(S32, S64),
+        "gpu_vs_oracle": snr_errors(S_gpu, S32),
+        "note": ("float64 recomputation: same plan, merge tree and float32 row map; "
+                 "downsampling weights, FFA sums and S/N in float64")}
+    # config 2 on the host
+    x2, kw2, P2, F2, S2, gpu_ms = c2
+    t0 = time.perf_counter()
+    _, _, Po, Fo, So = O.ffa_search(x2, 64e-6, **kw2)
+    out["c2"] = {"oracle_s": round(time.perf_counter() - t0, 2),
+                 "periods_bit_exact": bool(np.array_equal(P2, Po) and np.array_equal(F2, Fo)),
+                 "snr_within_1e-5rel_1e-4abs": bool(np.allclose(S2, So, rtol=1e-5, atol=1e-4)),
+                 "snr_max_abs_diff": float(np.max(np.abs(S2 - So)))}
+    out["c2"]["gpu_over_oracle"] = round(out["c2"]["oracle_s"] * 1e3 / gpu_ms, 1)
+    # find_peaks: host path of the reference (S/N copy + numpy) on two trials
+    finders, dev_ms, tobs = peaks_ctx
+    t0 = time.perf_counter()
+    same = True
+    for i in range(2):
+        P, F = batch.grid(i)
+        S = outs[i].cpu().numpy()
+        want, _ = OP.find_peaks(P, F, batch.widths[i], S, tobs)
+        got, _ = finders[int(bmin[i])].find(outs[i])
+        same = same and [tuple(p) for p in got] == want
+    out["find_peaks_host_ms_per_trial"] = round((time.perf_counter() - t0) / 2 * 1e3 - dev_ms, 2)
+    out["find_peaks_same_as_host_path"] = bool(same)
+    return out
 
 
 def fwd_bwd_timing(torch, _lib, reps=5):
@@ -225,11 +334,9 @@ def c5_timing(torch, _lib, seed, adam_steps=50, nseries=128, nsamp=2 ** 21, tsam
             "optimised_beats_running_median": int((hist[-1] > rm).sum())}
 
 
-def peak_timing(torch, batch, outs, bmin, tsamp, nsamp, ntrials=16, ncpu=2):
+def peak_timing(torch, batch, outs, bmin, tsamp, nsamp, ntrials=16):
     """find_peaks on the first `ntrials` S/N arrays of the batch with the
-    device path (PeakFinder per plan), and on `ncpu` of them with the host
-    path (device-to-host copy + the numpy restatement of the reference)."""
-    from oracle import peaks as OP
+    device path (one PeakFinder per plan); the S/N never leave the device."""
     from riptide_amd.peak_detection import PeakFinder
     tobs = nsamp * tsamp
     finders = {}
@@ -254,18 +361,74 @@ def peak_timing(torch, batch, outs, bmin, tsamp, nsamp, ntrials=16, ncpu=2):
         npk += len(pk)
     torch.cuda.synchronize()
     dev_ms = (time.perf_counter() - t0) / ntrials * 1e3
-    t0 = time.perf_counter()
-    same = True
-    for i in range(ncpu):
-        P, F = batch.grid(i)
-        S = outs[i].cpu().numpy()
-        want, _ = OP.find_peaks(P, F, batch.widths[i], S, tobs)
-        got, _ = finders[int(bmin[i])].find(outs[i])
-        same = same and [tuple(p) for p in got] == want
-    host_ms = (time.perf_counter() - t0) / ncpu * 1e3 - dev_ms
-    return {"trials": ntrials, "device_ms_per_trial": round(dev_ms, 2),
-            "host_path_ms_per_trial": round(host_ms, 2), "plan_setup_ms": round(t_plan * 1e3, 1),
-            "peaks_per_trial": round(npk / ntrials, 1), "same_peaks_as_host_path": bool(same)}
+    return ({"trials": ntrials, "device_ms_per_trial": round(dev_ms, 2),
+             "plan_setup_ms": round(t_plan * 1e3, 1),
+             "peaks_per_trial": round(npk / ntrials, 1)}, (finders, dev_ms, tobs))
+
+
+def d2h_timing(torch, outs, ntrials=8):
+    """Copy of the S/N arrays of `ntrials` trials to pinned host memory (what
+    a host consumer of the periodograms would pay; the batch itself leaves
+    them in HBM)."""
+    host = [torch.empty(outs[i].shape, dtype=outs[i].dtype, pin_memory=True)
+            for i in range(ntrials)]
+    for i in range(ntrials):
+        host[i].copy_(outs[i], non_blocking=True)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for i in range(ntrials):
+        host[i].copy_(outs[i], non_blocking=True)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    nbytes = sum(outs[i].numel() * 4 for i in range(ntrials))
+    return {"trials": ntrials, "ms_per_trial": round(ms / ntrials, 3),
+            "mb_per_trial": round(nbytes / ntrials / 1e6, 2),
+            "gb_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
+def c2_timing(torch, _lib, reps=3):
+    """Config 2 on the GPU: one 2^23-sample series at 64 us (white + red noise
+    of PSD index -1.5 equal to the white level at 0.1 Hz, pulse trains 0.2371 s
+    / 3 % / S/N 12 and 4.9133 s / 1 % / S/N 25), dereddening 4 s,
+    normalisation, periodogram over 0.1-10 s with bins 480-520, one series
+    per call (the drop-in ffa_search path).  Returns the host input and the
+    GPU results for the oracle's check, and the median milliseconds."""
+    from riptide_amd import libcpp
+    from riptide_amd import synthetic
+    from riptide_amd.ffautils import generate_width_trials
+    tsamp = 64e-6
+    kw = dict(period_min=0.1, period_max=10.0, bins_min=480, bins_max=520)
+    x = synthetic.series(2 ** 23, tsamp, seed=2, red_index=-1.5,
+                         pulses=[(0.2371, 0.03, 12.0), (4.9133, 0.01, 25.0)])
+    n = x.size
+    d = _lib.to_device(x)
+    t1, t2 = _lib.empty((n,)), _lib.empty((n,))
+    w = generate_width_trials(480)
+
+    def run():
+        _lib.call("rt_deredden", _lib.ptr(d), n, int(round(4.0 / tsamp)), 101, _lib.ptr(t1),
+                  _lib.stream())
+        _lib.call("rt_normalise", _lib.ptr(t1), n, _lib.ptr(t2), _lib.stream())
+        return libcpp.periodogram_device(t2, n, tsamp, w, kw["period_min"], kw["period_max"],
+                                         480, 520)
+
+    run()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        P, F, S = run()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = sorted(times)[reps // 2]
+    kw["rmed_width"] = 4.0
+    return (x, kw, _lib.to_numpy(P), _lib.to_numpy(F).view(np.uint32), _lib.to_numpy(S), ms)
 
 
 def main():
@@ -358,25 +521,13 @@ def main():
         phases = batch.run_timed()
         log(f"phases: {phases}")
 
-        # parity spot check of series 0 against the CPU oracle
-        from oracle import oracle as O
-        x0 = xs[0].cpu().numpy()
-        _, _, Po, Fo, So = O.ffa_search(x0, tsamp, period_min=pmin, period_max=pmax,
-                                        bins_min=int(bmin[0]), bins_max=int(bmax[0]))
-        S0 = outs[0].cpu().numpy()
-        P0, F0 = batch.grid(0)
-        parity = {
-            "series": 0,
-            "periods_bit_exact": bool(np.array_equal(P0, Po) and np.array_equal(F0, Fo)),
-            "snr_max_abs_diff": float(np.max(np.abs(S0 - So))),
-            "snr_within_1e-5rel_1e-4abs": bool(np.allclose(S0, So, rtol=1e-5, atol=1e-4)),
-        }
-        log(f"parity: {parity}")
+        # the S/N stay in HBM; what copying them to the host would cost
+        d2h = d2h_timing(torch, outs)
+        log(f"S/N device-to-host copy: {d2h}")
 
         # peak detection on the batch's S/N, left on the device (the consumer
-        # of the periodograms): device path vs the host path of the
-        # reference (copy of the S/N + numpy find_peaks, here the oracle)
-        peaks_info = peak_timing(torch, batch, outs, bmin, tsamp, args.nsamp)
+        # of the periodograms)
+        peaks_info, peaks_ctx = peak_timing(torch, batch, outs, bmin, tsamp, args.nsamp)
         log(f"find_peaks: {peaks_info}")
 
         # plain schedule: one series at a time, per-step launches, no graph.
@@ -469,6 +620,25 @@ def main():
         b3.close()
         del x3, o3, b3
 
+    # config 2: one 2^23-sample series through the drop-in path
+    with torch.cuda.stream(stream):
+        c2 = c2_timing(torch, _lib)
+    c2_info = {"workload": ("one series of 2^23 samples at 64 us, white + red noise, pulses at "
+                            "0.2371 s and 4.9133 s; dereddening, normalisation, periodogram P "
+                            "0.1-10 s, bins 480-520 (2,652,094 periods x 12 widths)"),
+               "gpu_ms": round(c2[5], 3)}
+    log(f"config 2: {c2_info}")
+    # config 1 series for the float64 error statement (GPU S/N now, the
+    # recomputation in the CPU leg)
+    from riptide_amd import synthetic
+    from riptide_amd.search import ffa_search
+    from riptide_amd.time_series import TimeSeries
+    kw1 = dict(period_min=0.5, period_max=2.0, bins_min=240, bins_max=260)
+    x1 = synthetic.series(2 ** 20, 256e-6, seed=1, red_index=None,
+                          pulses=[(1.3711, 0.02, 20.0)])
+    _, pg1 = ffa_search(TimeSeries(x1, 256e-6), rmed_width=4.0, **kw1)
+    c1 = (x1, 256e-6, kw1, pg1.snrs)
+
     # differentiable search (config 4 shape): forward vs forward + adjoint
     grad = fwd_bwd_timing(torch, _lib)
     log(f"fwd+bwd: {grad}")
@@ -499,11 +669,20 @@ def main():
             tj = json.load(f)
         traffic = int(tj["hbm_bytes_per_group"]) if "hbm_bytes_per_group" in tj else None
 
-    cpu = None
+    # CPU-baseline leg: the oracle timed on the host cores, and the checks of
+    # the GPU results made with the same oracle
+    cpu = checks = None
     if not args.no_cpu:
         t0 = time.time()
         cpu = cpu_baseline(args.nsamp, tsamp)
         log(f"cpu baseline: {cpu} ({time.time() - t0:.1f} s)")
+        t0 = time.time()
+        checks = oracle_checks(batch, xs, outs, coin, bmin, bmax, tsamp, pmin, pmax, c1, c2,
+                               peaks_ctx)
+        log(f"oracle checks: {checks} ({time.time() - t0:.1f} s)")
+        c2_info.update(checks.pop("c2"))
+        peaks_info["host_path_ms_per_trial"] = checks.pop("find_peaks_host_ms_per_trial")
+        peaks_info["same_peaks_as_host_path"] = checks.pop("find_peaks_same_as_host_path")
 
     line = {
         "metric": "DM trials/sec (2^22-sample ffa_search), batched vs plain, 1 GPU; fwd+bwd ms",
@@ -538,9 +717,14 @@ def main():
                                "the fastest kept, and weighted by the batch's class mix")},
         "batched_over_plain": round(trials_per_s / plain_tps, 3),
         "phases_ms": {k: round(v, 3) if isinstance(v, float) else v for k, v in phases.items()},
-        "parity": parity,
+        "parity": checks["parity"] if checks else None,
+        "snr_vs_fp64": checks["snr_vs_fp64"] if checks else None,
+        "snr_d2h": d2h,
+        "c2_single_series": c2_info,
         "roofline": {
             "bound": "hbm",
+            "binding_resource": ("LDS bandwidth and latency (merge levels, S/N windows), not "
+                                 "HBM: see roofline.lds; HBM is the contract figure"),
             "kernel": ("ffa_batch_kernel (fold + FFA merge levels + boxcar S/N); figures per "
                        "group of series: its launches, one per kernel variant"),
             "achieved": round(achieved, 1),
@@ -555,7 +739,12 @@ def main():
             "launches_per_group": round(launches / groups, 3),
             "lds": {"achieved": round(lds_achieved, 1), "peak": LDS_PEAK_GBS, "unit": "GB/s",
                     "frac": round(lds_achieved / LDS_PEAK_GBS, 4),
-                    "bytes": "8 B per FFA float add"},
+                    "peak_measured": round(LDS_MEASURED_GBS, 1),
+                    "frac_measured": round(lds_achieved / LDS_MEASURED_GBS, 4),
+                    "bytes": "8 B per FFA float add",
+                    "peaks": ("nominal 128 B/clk/CU; measured ds_read_b32 rate of gfx950 "
+                              "169 B/clk/CU (profiles/lds_read_rates__gfx950.md); 256 CU at "
+                              "2.4 GHz")},
         },
         "heterogeneous_batch": c3,
         "find_peaks": peaks_info,

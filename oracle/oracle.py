@@ -57,6 +57,10 @@ def _bind(lib, prefix):
         "periodogram", ctypes.c_int, _f32p, _sz, ctypes.c_double, _u64p, _sz,
         ctypes.c_double, ctypes.c_double, _sz, _sz, _f64p, _u32p, _f32p)
     b["running_median"] = fn("running_median", ctypes.c_int, _f32p, _sz, _sz, _f32p)
+    if prefix == "or_":
+        b["periodogram_f64"] = fn(
+            "periodogram_f64", ctypes.c_int, _f32p, _sz, ctypes.c_double, _u64p, _sz,
+            ctypes.c_double, ctypes.c_double, _sz, _sz, _f64p)
     return b
 
 
@@ -120,6 +124,16 @@ class CLib:
             data, data.size, tsamp, widths, widths.size, pmin, pmax, bmin, bmax,
             periods, foldbins, snrs))
         return periods, foldbins, snrs
+
+    def periodogram_f64(self, data, tsamp, widths, pmin, pmax, bmin, bmax):
+        """The periodogram S/N recomputed in float64 (C restatement only)."""
+        data = np.ascontiguousarray(data, dtype=np.float32)
+        widths = np.ascontiguousarray(widths, dtype=np.uint64)
+        L = self.periodogram_length(data.size, tsamp, pmin, pmax, bmin, bmax)
+        snrs = np.empty((L, widths.size), dtype=np.float64)
+        self._check(self.b["periodogram_f64"](data, data.size, tsamp, widths, widths.size,
+                                              pmin, pmax, bmin, bmax, snrs))
+        return snrs
 
     def running_median(self, x, width):
         x = np.ascontiguousarray(x, dtype=np.float32)

@@ -378,6 +378,104 @@ int or_periodogram(const float* data, size_t size, double tsamp, const uint64_t*
 }
 
 /* ------------------------------------------------------------------------ */
+/* Float64 recomputation of the periodogram (for error statements only)      */
+/* ------------------------------------------------------------------------ */
+
+/* The same search as or_periodogram with every value held in float64: the
+ * downsampling weights (imin + 1 - start, end - imax) are not rounded to
+ * float32, the FFA sums and the boxcar S/N (h, b, prefix sums) are float64.
+ * The plan, the merge tree and its float32 row map are the algorithm's
+ * definition and stay as they are.  Output: float64 S/N, plan order. */
+static void ffa_rec_f64(const double* in, double* temp, double* out, size_t m, size_t p)
+{
+    if (m == 1) {
+        memcpy(out, in, p * sizeof(double));
+        return;
+    }
+    const size_t mh = m >> 1;
+    const size_t mt = m - mh;
+    ffa_rec_f64(in, out, temp, mh, p);
+    ffa_rec_f64(in + mh * p, out + mh * p, temp + mh * p, mt, p);
+    const double* head = temp;
+    const double* tail = temp + mh * p;
+    const float kh = ((float)mh - 1.0f) / ((float)m - 1.0f);
+    const float kt = ((float)mt - 1.0f) / ((float)m - 1.0f);
+    for (size_t s = 0; s < m; ++s) {
+        const size_t h = m == 2 ? 0 : (size_t)(kh * (float)s + 0.5f);
+        const size_t t = m == 2 ? 0 : (size_t)(kt * (float)s + 0.5f);
+        const size_t sh = (s - t) % p;
+        for (size_t c = 0; c < p; ++c)
+            out[s * p + c] = head[h * p + c] + tail[t * p + (c + sh) % p];
+    }
+}
+
+int or_periodogram_f64(const float* data, size_t size, double tsamp, const uint64_t* widths,
+                       size_t nw, double pmin, double pmax, size_t bmin, size_t bmax,
+                       double* snr)
+{
+    int st = or_periodogram_check(size, tsamp, pmin, pmax, bmin, bmax);
+    if (st) return st;
+    const double ds_ini = pmin / (tsamp * bmin);
+    const double ds_geo = (bmax + 1.0) / bmin;
+    const size_t nds = (size_t)ceil(log(pmax / pmin) / log(ds_geo));
+    const size_t cap = size;
+    double* x = (double*)malloc(cap * sizeof(double));
+    double* tmp = (double*)malloc(cap * sizeof(double));
+    double* ffa = (double*)malloc(cap * sizeof(double));
+    double* cps = (double*)malloc((2 * bmax + 1) * sizeof(double));
+    for (size_t ids = 0; ids < nds; ++ids) {
+        const double f = ds_ini * pow(ds_geo, ids);
+        const double tau = f * tsamp;
+        const double pmax_samples = pmax / tau;
+        const size_t n = or_downsampled_size(size, f);
+        if (f != 1) {
+            for (size_t k = 0; k < n; ++k) {
+                const double start = k * f;
+                const double end = start + f;
+                const size_t imin = (size_t)floor(start);
+                const size_t imax = (size_t)fmin(floor(end), size - 1.0);
+                double acc = ((imin + 1) - start) * data[imin];
+                for (size_t i = imin + 1; i < imax; ++i) acc += data[i];
+                acc += (end - imax) * data[imax];
+                x[k] = acc;
+            }
+        } else {
+            for (size_t k = 0; k < n; ++k) x[k] = data[k];
+        }
+        const size_t bstop = min3(bmax, n, (size_t)pmax_samples);
+        for (size_t bins = bmin; bins <= bstop; ++bins) {
+            const size_t rows = n / bins;
+            const double stdnoise = sqrt(rows * or_downsampled_variance(size, f));
+            const double pceil = fmin(pmax_samples, bins + 1.0);
+            const size_t ce = or_ceilshift(rows, bins, pceil);
+            const size_t rows_eval = rows < ce ? rows : ce;
+            ffa_rec_f64(x, tmp, ffa, rows, bins);
+            for (size_t r = 0; r < rows_eval; ++r) {
+                const double* prof = ffa + r * bins;
+                cps[0] = 0.0;
+                for (size_t j = 0; j < 2 * bins; ++j) cps[j + 1] = cps[j] + prof[j % bins];
+                const double total = cps[bins];
+                for (size_t iw = 0; iw < nw; ++iw) {
+                    const size_t w = widths[iw];
+                    const double h = sqrt((bins - w) / (double)(bins * w));
+                    const double b = w / (double)(bins - w) * h;
+                    double best = -INFINITY;
+                    for (size_t i = 0; i < bins; ++i)
+                        if (cps[i + w] - cps[i] > best) best = cps[i + w] - cps[i];
+                    snr[r * nw + iw] = ((h + b) * best - b * total) / stdnoise;
+                }
+            }
+            snr += rows_eval * nw;
+        }
+    }
+    free(x);
+    free(tmp);
+    free(ffa);
+    free(cps);
+    return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Running median: running_median.hpp:13-132                                 */
 /* ------------------------------------------------------------------------ */
 

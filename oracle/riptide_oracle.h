@@ -39,6 +39,10 @@ int or_periodogram(const float* data, size_t size, double tsamp, const uint64_t*
                    size_t nw, double pmin, double pmax, size_t bmin, size_t bmax,
                    double* periods, uint32_t* foldbins, float* snr);
 
+int or_periodogram_f64(const float* data, size_t size, double tsamp, const uint64_t* widths,
+                       size_t nw, double pmin, double pmax, size_t bmin, size_t bmax,
+                       double* snr);
+
 int or_running_median(const float* data, size_t size, size_t width, float* out);
 
 #ifdef __cplusplus

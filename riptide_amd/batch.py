@@ -139,7 +139,8 @@ def ffa_search_batch(tseries_list, period_min=1.0, period_max=30.0, fpmin=8, bin
                     deredden=deredden, rmed_width=rmed_width, rmed_minpts=rmed_minpts,
                     already_normalised=already_normalised)
     try:
-        inputs = [_lib.to_device(ts.data) for ts in tseries_list]
+        inputs = [ts.data_device if ts.data_device is not None else _lib.to_device(ts.data)
+                  for ts in tseries_list]
         outputs = b.allocate_outputs()
         b.bind(inputs, outputs)
         b.run(use_graph=use_graph)
@@ -147,8 +148,9 @@ def ffa_search_batch(tseries_list, period_min=1.0, period_max=30.0, fpmin=8, bin
         out = []
         for i, ts in enumerate(tseries_list):
             P, F = b.grid(i)
+            # S/N left in HBM, copied to the host when first read
             out.append(Periodogram(b.widths[i].astype(np.int64), P.copy(), F.copy(),
-                                   _lib.to_numpy(outputs[i]), metadata=ts.metadata))
+                                   outputs[i], metadata=ts.metadata))
         return out
     finally:
         b.close()

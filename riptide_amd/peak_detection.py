@@ -217,7 +217,9 @@ def find_peaks(pgram, smin=6.0, segwidth=5.0, nstd=6.0, minseg=10, polydeg=2, cl
     a numpy array (copied to the device) or a CUDA tensor."""
     finder = PeakFinder(pgram.periods, pgram.foldbins, pgram.widths, pgram.tobs,
                         segwidth=segwidth)
-    snrs = pgram.snrs
+    snrs = getattr(pgram, "snrs_device", None)
+    if snrs is None:
+        snrs = pgram.snrs
     if isinstance(snrs, np.ndarray):
         snrs = _lib.to_device(np.ascontiguousarray(snrs, np.float32))
     return finder.find(snrs, dm=pgram.metadata["dm"], smin=smin, nstd=nstd, minseg=minseg,

@@ -21,14 +21,35 @@ class TimeSeries(object):
             self._data = np.asarray(data, dtype=np.float32).copy()
         else:
             self._data = np.asarray(data, dtype=np.float32)
+        self._dev = None
         self._tsamp = float(tsamp)
         self.metadata = Metadata(metadata) if metadata is not None else Metadata({})
         # time_series.py:49-51
         self.metadata["tobs"] = self.length
 
+    @classmethod
+    def from_device(cls, tensor, tsamp, metadata=None):
+        """A TimeSeries whose samples stay in a float32 device tensor until
+        ``data`` is first read."""
+        ts = cls.__new__(cls)
+        ts._data = None
+        ts._dev = tensor
+        ts._tsamp = float(tsamp)
+        ts.metadata = Metadata(metadata) if metadata is not None else Metadata({})
+        ts.metadata["tobs"] = ts.length
+        return ts
+
     @property
     def data(self):
+        if self._data is None:
+            self._data = self._dev.cpu().numpy()
         return self._data
+
+    @property
+    def data_device(self):
+        """The device tensor of the samples while the host copy has not been
+        made (None otherwise)."""
+        return self._dev if self._data is None else None
 
     @property
     def tsamp(self):
@@ -130,7 +151,7 @@ class TimeSeries(object):
 
     @property
     def nsamp(self):
-        return self.data.size
+        return self._data.size if self._data is not None else int(self._dev.numel())
 
     @property
     def length(self):

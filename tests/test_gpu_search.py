@@ -237,3 +237,29 @@ def test_periodogram_default_width_counts(oracle, bmin, N, shape):
     Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
     assert np.array_equal(P, Po) and np.array_equal(F, Fo)
     np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)
+
+
+def test_ffa_search_results_stay_on_device(oracle):
+    """ffa_search leaves the searched series and the S/N in HBM: no host copy
+    until they are read, and the copies then equal the oracle's results."""
+    import riptide_amd as rt
+    N, tsamp = 2 ** 17, 256e-6
+    x = _pulsar(N, tsamp, 0.9, 0.03, 15.0, seed=8)
+    ts = rt.TimeSeries(x, tsamp)
+    tsdr, pg = rt.ffa_search(ts, period_min=0.5, period_max=2.0, bins_min=240, bins_max=260,
+                             rmed_width=4.0)
+    assert pg.snrs_device is not None and not pg.snrs_on_host
+    assert tsdr.data_device is not None and tsdr.nsamp == N
+    assert tsdr.metadata["tobs"] == ts.length
+    xo, _, Po, Fo, So = oracle.ffa_search(x, tsamp, period_min=0.5, period_max=2.0,
+                                          bins_min=240, bins_max=260, rmed_width=4.0)
+    peaks, _ = rt.find_peaks(pg)  # reads the device copy
+    assert not pg.snrs_on_host
+    np.testing.assert_allclose(pg.snrs, So, rtol=SNR_RTOL, atol=SNR_ATOL)
+    assert pg.snrs_on_host and tsdr.data_device is not None
+    np.testing.assert_allclose(tsdr.data, xo, rtol=0, atol=4e-6)
+    assert tsdr.data_device is None
+    # a device-backed series searched again: no host round trip needed
+    _, pg2 = rt.ffa_search(tsdr, period_min=0.5, period_max=2.0, bins_min=240, bins_max=260,
+                           deredden=False, already_normalised=True)
+    np.testing.assert_array_equal(pg2.snrs, pg.snrs)

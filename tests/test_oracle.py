@@ -480,3 +480,27 @@ def test_oracle_matches_golden_fixtures():
     assert np.array_equal(P, G["pgram_periods"])
     assert np.array_equal(F, G["pgram_foldbins"])
     assert np.array_equal(S, G["pgram_snrs"])
+
+
+def test_periodogram_f64_recomputation(oracle):
+    """The float64 recomputation used for error statements agrees with the
+    float32 oracle to float32 accuracy and with the independent float64 torch
+    reimplementation of the path (tests/fp64_periodogram.py, all transforms)
+    to float64 accuracy, apart from the float32 rounding of the downsampling
+    edge weights that the latter keeps (~1e-8 relative)."""
+    import torch
+    from tests import fp64_periodogram as R
+    rng = np.random.default_rng(21)
+    N, tsamp, pr = 20_000, 1e-3, (0.3, 0.6, 30, 33)
+    x = oracle.normalise(rng.standard_normal(N).astype(np.float32) + 0.5 * np.sin(np.arange(N)))
+    w = oracle.generate_width_trials(30)
+    S64 = oracle.oracle().periodogram_f64(x, tsamp, w, *pr)
+    _, _, S32 = oracle.oracle().periodogram(x, tsamp, w, *pr)
+    assert S64.shape == S32.shape and S64.dtype == np.float64
+    np.testing.assert_allclose(S32, S64, rtol=1e-5, atol=1e-5)
+    rows = R.periodogram_rows(torch.tensor(x, dtype=torch.float64), tsamp, [int(v) for v in w],
+                              *pr, keep=lambda t: True, normalise_input=False)
+    T = torch.cat([rows[k] for k in sorted(rows)]).numpy()
+    np.testing.assert_allclose(S64, T, rtol=1e-6, atol=1e-6)
+    # the float32 path's error is of float32 size, the float64 one's far below
+    assert np.abs(S32 - S64).max() > 10 * np.abs(S64 - T).max() or np.abs(S32 - S64).max() < 1e-6

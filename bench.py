@@ -447,6 +447,8 @@ def main():
     ap.add_argument("--seed", type=int, default=20260)
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the heterogeneous batch (lengths log-uniform 2^21-2^23)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="only the batched run and its phase timings (for profiling)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain copy kernel over 1 GiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -520,6 +522,13 @@ def main():
         # per-phase event timing of one eager run (same kernels, same stream)
         phases = batch.run_timed()
         log(f"phases: {phases}")
+
+        if args.batch_only:
+            print(json.dumps({"workload": f"batched ffa_search, {args.trials} x {args.nsamp}",
+                              "trials_per_s": round(trials_per_s, 2),
+                              "ms_per_step": round(batched_ms, 3), "phases_ms": phases,
+                              "stats": st}), flush=True)
+            return
 
         # the S/N stay in HBM; what copying them to the host would cost
         d2h = d2h_timing(torch, outs)

@@ -175,6 +175,38 @@ int rt_periodogram_backward(const float* d_in, int64_t size, double tsamp,
                             double period_max, int64_t bins_min, int64_t bins_max,
                             const float* d_gsnr, float* d_gin, void* stream);
 
+/* ---- Batched differentiable periodogram (no counterpart in the reference,
+ * which has no gradients): nseries series of `size` samples that share one
+ * search plan, stored row-major (series i at d + i*size).  The forward
+ * periodogram of the normalised rows runs as a batch (rt_batch_*); these
+ * entry points normalise the rows and take the gradient back to the
+ * un-normalised input in a fixed number of launches for the whole batch.
+ * Results are bit-identical to rt_normalise_ex + rt_periodogram_backward +
+ * rt_normalise_backward on each series. ---------------------------------- */
+
+/* Plans the batch (widths 1 to 64 of them, each > 0 and < bins_min). */
+int rt_grad_batch_create(int64_t nseries, int64_t size, double tsamp, const uint64_t* h_widths,
+                         int64_t nw, double period_min, double period_max, int64_t bins_min,
+                         int64_t bins_max, void** handle);
+
+/* Trial periods per series. */
+int rt_grad_batch_length(void* handle, int64_t* length);
+
+/* d_y = every row of d_x normalised (time_series.py:67-93); h_inv_std (may
+ * be NULL) receives 1/std of every row, in float32 as the one-series
+ * adjoint takes it.  Waits for the work. */
+int rt_grad_batch_normalise(void* handle, const float* d_x, float* d_y, float* h_inv_std,
+                            void* stream);
+
+/* d_gx (nseries x size) = gradient of sum(d_gsnr * periodogram(d_y)) with
+ * respect to the un-normalised rows, given the normalised rows d_y and their
+ * h_inv_std; d_gsnr is nseries x length x nw.  Recomputes the forward of
+ * every (series, transform) pair that receives gradient.  Waits. */
+int rt_grad_batch_backward(void* handle, const float* d_y, const float* h_inv_std,
+                           const float* d_gsnr, float* d_gx, void* stream);
+
+int rt_grad_batch_destroy(void* handle);
+
 /* ---- Batched DM-trial search (no counterpart in the reference: it replaces
  * the reference's one-process-per-DM-trial pool, pipeline/worker_pool.py:35-45,
  * each worker calling ffa_search, search.py:10-93) -------------------------- */

@@ -62,6 +62,12 @@ _SIGNATURES = {
     "rt_batch_check": [_c_ptr, _c_ptr],
     "rt_batch_stats": [_c_ptr, _c_ptr],
     "rt_batch_destroy": [_c_ptr],
+    "rt_grad_batch_create": [_c_i64, _c_i64, _c_dbl, _c_ptr, _c_i64, _c_dbl, _c_dbl, _c_i64,
+                             _c_i64, _c_ptr],
+    "rt_grad_batch_length": [_c_ptr, _c_ptr],
+    "rt_grad_batch_normalise": [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr],
+    "rt_grad_batch_backward": [_c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr, _c_ptr],
+    "rt_grad_batch_destroy": [_c_ptr],
     "rt_fir_forward": [_c_ptr, _c_i64, _c_i64, _c_ptr, _c_i64, ctypes.c_int, _c_ptr, _c_ptr],
     "rt_fir_workspace_floats": [_c_i64, _c_i64, _c_i64, _c_ptr],
     "rt_fir_grad_taps": [_c_ptr, _c_ptr, _c_i64, _c_i64, _c_i64, _c_ptr, _c_ptr, _c_ptr],

@@ -11,6 +11,14 @@
 
 namespace rt {
 
+static inline int grid_of(int64_t n, int threads)
+{
+    int64_t g = (n + threads - 1) / threads;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (int)g;
+}
+
 // ---------------------------------------------------------------------------
 // Boxcar S/N backward (snr.hpp:37-55): snr_w = ((h+b)*dmax_w - b*total)/std,
 // dmax_w = cps[i*+w] - cps[i*] = sum of x over the window [i*+1, i*+w]
@@ -18,16 +26,17 @@ namespace rt {
 // One wavefront per row; rows with an all-zero upstream gradient get zeros.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void snr_backward_kernel(
+// Rows [block * waves, (block + 1) * waves) of one transform, one row per
+// wavefront; `lds` holds waves * bins floats plus the window tables.
+__device__ __forceinline__ void snr_backward_block(
     const float* __restrict__ x, int rows, int bins, const float* __restrict__ gsnr,
     const int32_t* __restrict__ widths, const float* __restrict__ hb, int nw, float stdnoise,
-    float* __restrict__ gx)
+    float* __restrict__ gx, int block, float* lds)
 {
-    extern __shared__ float lds[];
     const int wave = threadIdx.x / kWave;
     const int lane = threadIdx.x & (kWave - 1);
     const int nwaves = blockDim.x / kWave;
-    const int r = blockIdx.x * nwaves + wave;
+    const int r = block * nwaves + wave;
     float* cps = lds + (size_t)wave * bins;
     // per wave: window start and height increment of each width
     int* s_start = reinterpret_cast<int*>(lds + (size_t)nwaves * bins) + wave * kSnrBwdMaxW;
@@ -91,6 +100,15 @@ __global__ __launch_bounds__(256) void snr_backward_kernel(
     }
 }
 
+__global__ __launch_bounds__(256) void snr_backward_kernel(
+    const float* __restrict__ x, int rows, int bins, const float* __restrict__ gsnr,
+    const int32_t* __restrict__ widths, const float* __restrict__ hb, int nw, float stdnoise,
+    float* __restrict__ gx)
+{
+    extern __shared__ float lds[];
+    snr_backward_block(x, rows, bins, gsnr, widths, hb, nw, stdnoise, gx, blockIdx.x, lds);
+}
+
 // ---------------------------------------------------------------------------
 // FFA adjoint, one merge level (transforms.hpp:13-27 transposed): the
 // forward adds head[h(s)] and roll(tail[t(s)], -(s - t(s))) into out[s]; the
@@ -98,12 +116,13 @@ __global__ __launch_bounds__(256) void snr_backward_kernel(
 // g_child has the state layout of depth d+1, g_parent of depth d.
 // ---------------------------------------------------------------------------
 
-__global__ void ffa_adjoint_level_kernel(const float* __restrict__ gp, float* __restrict__ gc,
-                                         int64_t rows, int64_t cols, int depth)
+__device__ __forceinline__ void ffa_adjoint_level_range(const float* __restrict__ gp,
+                                                        float* __restrict__ gc, int64_t rows,
+                                                        int64_t cols, int depth, int64_t first,
+                                                        int64_t stride)
 {
     const int64_t total = rows * cols;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t e = first; e < total; e += stride) {
         const int64_t r = e / cols;
         const int64_t c = e - r * cols;
         // parent node at `depth` containing row r
@@ -160,6 +179,14 @@ __global__ void ffa_adjoint_level_kernel(const float* __restrict__ gp, float* __
     }
 }
 
+__global__ void ffa_adjoint_level_kernel(const float* __restrict__ gp, float* __restrict__ gc,
+                                         int64_t rows, int64_t cols, int depth)
+{
+    ffa_adjoint_level_range(gp, gc, rows, cols, depth,
+                            (int64_t)blockIdx.x * blockDim.x + threadIdx.x,
+                            (int64_t)gridDim.x * blockDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // Downsampling adjoint (downsample.hpp:58-78 transposed): every input sample
 // collects the weights it had in the (at most three) outputs that read it.
@@ -183,24 +210,32 @@ __global__ void downsample_index_kernel(int64_t n, uint64_t mf, int ef, int64_t 
 // it: those lie among k = floor(i/f) - 2 .. floor(i/f) + 3 (an output spans
 // at least one input, its ends are floor(k*f) and floor(k*f + f) to within
 // one sample, and i*(1/f) in float32 is within one of i/f).
+__device__ __forceinline__ float downsample_adjoint_at(const float* __restrict__ gout, int64_t n,
+                                                      const int4* __restrict__ idx, float inv_f,
+                                                      int64_t i)
+{
+    int64_t k0 = (int64_t)((float)i * inv_f) - 2;
+    if (k0 < 0) k0 = 0;
+    float acc = 0.0f;
+    for (int64_t k = k0; k < k0 + 6 && k < n; ++k) {
+        const int4 ix = idx[k];
+        if (i < ix.x || i > ix.y) continue;
+        float w = 0.0f;
+        if (i == ix.x) w += __int_as_float(ix.z);
+        if (i == ix.y) w += __int_as_float(ix.w);
+        if (i > ix.x && i < ix.y) w += 1.0f;
+        acc += w * gout[k];
+    }
+    return acc;
+}
+
 __global__ void downsample_adjoint_kernel(const float* __restrict__ gout, int64_t n,
                                           const int4* __restrict__ idx, float inv_f, int64_t N,
                                           float* __restrict__ gin, int accumulate)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t k0 = (int64_t)((float)i * inv_f) - 2;
-        if (k0 < 0) k0 = 0;
-        float acc = 0.0f;
-        for (int64_t k = k0; k < k0 + 6 && k < n; ++k) {
-            const int4 ix = idx[k];
-            if (i < ix.x || i > ix.y) continue;
-            float w = 0.0f;
-            if (i == ix.x) w += __int_as_float(ix.z);
-            if (i == ix.y) w += __int_as_float(ix.w);
-            if (i > ix.x && i < ix.y) w += 1.0f;
-            acc += w * gout[k];
-        }
+        const float acc = downsample_adjoint_at(gout, n, idx, inv_f, i);
         gin[i] = accumulate ? gin[i] + acc : acc;
     }
 }
@@ -221,13 +256,14 @@ __global__ void add_prefix_kernel(const float* __restrict__ gout, int64_t n, int
 //   dx = (g - mean(g) - y * mean(g*y)) / s
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void normalise_adjoint_partial_kernel(
-    const float* __restrict__ g, const float* __restrict__ y, int64_t N, float* __restrict__ part)
+__device__ __forceinline__ void normalise_adjoint_partial(
+    const float* __restrict__ g, const float* __restrict__ y, int64_t N, float* __restrict__ part,
+    int block, int nblocks)
 {
     __shared__ float sh[4][4];
     float a = 0.0f, al = 0.0f, b = 0.0f, bl = 0.0f;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (int64_t)gridDim.x * blockDim.x) {
+    for (int64_t i = (int64_t)block * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)nblocks * blockDim.x) {
         ff_add(a, al, g[i], 0.0f, a, al);
         const float p = g[i] * y[i];
         ff_add(b, bl, p, __fmaf_rn(g[i], y[i], -p), b, bl);
@@ -248,18 +284,25 @@ __global__ __launch_bounds__(256) void normalise_adjoint_partial_kernel(
             ff_add(a, al, sh[w][0], sh[w][1], a, al);
             ff_add(b, bl, sh[w][2], sh[w][3], b, bl);
         }
-        part[4 * blockIdx.x + 0] = a;
-        part[4 * blockIdx.x + 1] = al;
-        part[4 * blockIdx.x + 2] = b;
-        part[4 * blockIdx.x + 3] = bl;
+        part[4 * block + 0] = a;
+        part[4 * block + 1] = al;
+        part[4 * block + 2] = b;
+        part[4 * block + 3] = bl;
     }
+}
+
+__global__ __launch_bounds__(256) void normalise_adjoint_partial_kernel(
+    const float* __restrict__ g, const float* __restrict__ y, int64_t N, float* __restrict__ part)
+{
+    normalise_adjoint_partial(g, y, N, part, blockIdx.x, gridDim.x);
 }
 
 // The partial float-float sums are reduced by every block in parallel (lane
 // p of the block takes partials p, p + blockDim, ...), then applied.
-__global__ __launch_bounds__(256) void normalise_adjoint_apply_kernel(
+__device__ __forceinline__ void normalise_adjoint_apply(
     const float* __restrict__ g, const float* __restrict__ y, int64_t N,
-    const float* __restrict__ part, int nparts, float inv_std, float* __restrict__ gx)
+    const float* __restrict__ part, int nparts, float inv_std, float* __restrict__ gx, int block,
+    int nblocks)
 {
     __shared__ float sh[4][4];
     __shared__ float s_mg, s_mgy;
@@ -289,9 +332,16 @@ __global__ __launch_bounds__(256) void normalise_adjoint_apply_kernel(
     }
     __syncthreads();
     const float mg = s_mg, mgy = s_mgy;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
-         i += (int64_t)gridDim.x * blockDim.x)
+    for (int64_t i = (int64_t)block * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)nblocks * blockDim.x)
         gx[i] = (g[i] - mg - y[i] * mgy) * inv_std;
+}
+
+__global__ __launch_bounds__(256) void normalise_adjoint_apply_kernel(
+    const float* __restrict__ g, const float* __restrict__ y, int64_t N,
+    const float* __restrict__ part, int nparts, float inv_std, float* __restrict__ gx)
+{
+    normalise_adjoint_apply(g, y, N, part, nparts, inv_std, gx, blockIdx.x, gridDim.x);
 }
 
 // flags[t] = 1 if any of the rows_eval x nw S/N gradients of transform t
@@ -310,16 +360,252 @@ __global__ __launch_bounds__(256) void transform_need_kernel(const float* __rest
 }
 
 // ---------------------------------------------------------------------------
+// Batched backward over (series, transform) pairs and (series, step) groups
+// of series that share one search plan.  Every kernel runs the one-series
+// kernels' per-element arithmetic on the pair's or group's buffers, in the
+// same order, so the results are bit-identical to one-series backwards.
+// Blocks map to items through host-built tables: blk[b] = (item, local
+// block, blocks of the item, depth).
+// ---------------------------------------------------------------------------
+
+// The buffer a pair's gradient ends in after `levels` transposed merge levels.
+__device__ __forceinline__ const float* grad_pair_final(const GradPair& p, int levels)
+{
+    if (levels == 0) return p.A;
+    return ((levels - 1) & 1) ? p.C : p.B;
+}
+
+__global__ void flags_batch_kernel(const float* __restrict__ gsnr, int64_t series_stride,
+                                   const int2* __restrict__ rows, int ntr, int nw,
+                                   int* __restrict__ flags)
+{
+    const int t = blockIdx.x, s = blockIdx.y;
+    const int2 r = rows[t];
+    const float* g = gsnr + (size_t)s * series_stride + (size_t)r.x * nw;
+    const int64_t cnt = (int64_t)r.y * nw;
+    int any = 0;
+    for (int64_t e = threadIdx.x; e < cnt; e += blockDim.x) any |= g[e] != 0.0f;
+    any = __syncthreads_or(any);
+    if (threadIdx.x == 0) flags[(size_t)s * ntr + t] = any;
+}
+
+__global__ void ds_index_steps_kernel(const GradStep* __restrict__ steps,
+                                      const int4* __restrict__ blk, int64_t N)
+{
+    const int4 b = blk[blockIdx.x];
+    const GradStep st = steps[b.x];
+    for (int64_t k = (int64_t)b.y * blockDim.x + threadIdx.x; k < st.n;
+         k += (int64_t)b.z * blockDim.x) {
+        const DsIndex ix = ds_index_fast(k, st.mf, st.ef, N);
+        st.idx[k] = make_int4((int)ix.imin, (int)ix.imax, __float_as_int(ix.wmin),
+                              __float_as_int(ix.wmax));
+    }
+}
+
+__global__ void ds_values_groups_kernel(const GradGroup* __restrict__ groups,
+                                        const GradStep* __restrict__ steps,
+                                        const int4* __restrict__ blk)
+{
+    const int4 b = blk[blockIdx.x];
+    const GradGroup g = groups[b.x];
+    const GradStep st = steps[g.step];
+    for (int64_t k = (int64_t)b.y * blockDim.x + threadIdx.x; k < st.n;
+         k += (int64_t)b.z * blockDim.x) {
+        const int4 e = st.idx[k];
+        DsIndex ix;
+        ix.imin = e.x;
+        ix.imax = e.y;
+        ix.wmin = __int_as_float(e.z);
+        ix.wmax = __int_as_float(e.w);
+        g.ds[k] = ds_accumulate(g.x, ix);
+    }
+}
+
+// zeros in every pair's S/N-gradient rows (A), before the S/N backward
+__global__ void fill_pairs_kernel(const GradPair* __restrict__ pairs, const int4* __restrict__ blk)
+{
+    const int4 b = blk[blockIdx.x];
+    const GradPair p = pairs[b.x];
+    const int64_t n = (int64_t)p.rows * p.bins;
+    for (int64_t i = (int64_t)b.y * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)b.z * blockDim.x)
+        p.A[i] = 0.0f;
+}
+
+__global__ __launch_bounds__(256) void snr_backward_pairs_kernel(
+    const GradPair* __restrict__ pairs, const int4* __restrict__ blk,
+    const int32_t* __restrict__ widths, int nw)
+{
+    extern __shared__ float lds[];
+    const int4 b = blk[blockIdx.x];
+    const GradPair p = pairs[b.x];
+    snr_backward_block(p.T, p.rows_eval, p.bins, p.gsnr, widths, p.hb, nw, p.stdnoise, p.A, b.y,
+                       lds);
+}
+
+__global__ void ffa_adjoint_level_pairs_kernel(const GradPair* __restrict__ pairs,
+                                               const int4* __restrict__ blk)
+{
+    const int4 b = blk[blockIdx.x];
+    const GradPair p = pairs[b.x];
+    const int d = b.w;
+    const float* src = d == 0 ? p.A : (((d - 1) & 1) ? p.C : p.B);
+    float* dst = (d & 1) ? p.C : p.B;
+    ffa_adjoint_level_range(src, dst, p.rows, p.bins, d,
+                            (int64_t)b.y * blockDim.x + threadIdx.x, (int64_t)b.z * blockDim.x);
+}
+
+// gds of a (series, step) group: the fold transposes of its pairs' gradients
+// added in pair order, from zero
+__global__ void fold_groups_kernel(const GradGroup* __restrict__ groups,
+                                   const GradPair* __restrict__ pairs,
+                                   const GradStep* __restrict__ steps,
+                                   const int4* __restrict__ blk)
+{
+    const int4 b = blk[blockIdx.x];
+    const GradGroup g = groups[b.x];
+    const int64_t n = steps[g.step].n;
+    for (int64_t i = (int64_t)b.y * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)b.z * blockDim.x) {
+        float v = 0.0f;
+        for (int q = g.p0; q < g.p1; ++q) {
+            const GradPair& p = pairs[q];
+            const int64_t m = (int64_t)p.rows * p.bins;
+            v = v + (i < m ? grad_pair_final(p, p.H)[i] : 0.0f);
+        }
+        g.gds[i] = v;
+    }
+}
+
+// gradient of every series with respect to its (normalised) input: the
+// downsampling transposes of its groups added in step order, from zero;
+// series = blockIdx.y, its groups [g0[s], g0[s + 1])
+__global__ void ds_adjoint_series_kernel(const GradGroup* __restrict__ groups,
+                                         const GradStep* __restrict__ steps,
+                                         const int* __restrict__ g0, int64_t N,
+                                         float* __restrict__ gz)
+{
+    const int s = blockIdx.y;
+    const int ga = g0[s], gb = g0[s + 1];
+    float* out = gz + (size_t)s * N;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < N;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        float v = 0.0f;
+        for (int q = ga; q < gb; ++q) {
+            const GradGroup& g = groups[q];
+            const GradStep& st = steps[g.step];
+            const float c = st.idx ? downsample_adjoint_at(g.gds, st.n, st.idx, st.inv_f, i)
+                                   : (i < st.n ? g.gds[i] : 0.0f);
+            v = v + c;
+        }
+        out[i] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void normalise_adjoint_partial_batch_kernel(
+    const float* __restrict__ g, const float* __restrict__ y, int64_t N, float* __restrict__ part,
+    int nparts)
+{
+    const size_t s = blockIdx.y;
+    normalise_adjoint_partial(g + s * N, y + s * N, N, part + s * 4 * nparts, blockIdx.x,
+                              gridDim.x);
+}
+
+__global__ __launch_bounds__(256) void normalise_adjoint_apply_batch_kernel(
+    const float* __restrict__ g, const float* __restrict__ y, int64_t N,
+    const float* __restrict__ part, int nparts, const float* __restrict__ inv_std,
+    float* __restrict__ gx)
+{
+    const size_t s = blockIdx.y;
+    normalise_adjoint_apply(g + s * N, y + s * N, N, part + s * 4 * nparts, nparts, inv_std[s],
+                            gx + s * N, blockIdx.x, gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
-static inline int grid_of(int64_t n, int threads)
+hipError_t launch_flags_batch(const float* gsnr, int64_t series_stride, const int2* rows, int ntr,
+                              int nw, int nseries, int* flags, hipStream_t s)
 {
-    int64_t g = (n + threads - 1) / threads;
-    if (g < 1) g = 1;
-    if (g > 65536) g = 65536;
-    return (int)g;
+    if (ntr <= 0 || nseries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(flags_batch_kernel, dim3(ntr, nseries), dim3(256), 0, s, gsnr,
+                       series_stride, rows, ntr, nw, flags);
+    return hipGetLastError();
 }
+
+hipError_t launch_ds_index_steps(const GradStep* steps, const int4* blk, int nblk, int64_t N,
+                                 hipStream_t s)
+{
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ds_index_steps_kernel, dim3(nblk), dim3(256), 0, s, steps, blk, N);
+    return hipGetLastError();
+}
+
+hipError_t launch_ds_values_groups(const GradGroup* groups, const GradStep* steps,
+                                   const int4* blk, int nblk, hipStream_t s)
+{
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ds_values_groups_kernel, dim3(nblk), dim3(256), 0, s, groups, steps, blk);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_pairs(const GradPair* pairs, const int4* blk, int nblk, hipStream_t s)
+{
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_pairs_kernel, dim3(nblk), dim3(256), 0, s, pairs, blk);
+    return hipGetLastError();
+}
+
+hipError_t launch_snr_backward_pairs(const GradPair* pairs, const int4* blk, int nblk,
+                                     int max_bins, const int32_t* widths, int nw, hipStream_t s)
+{
+    if (nblk <= 0) return hipSuccess;
+    if (nw > kSnrBwdMaxW) return hipErrorInvalidValue;
+    const int waves = kGradSnrWaves;
+    const size_t lds = (size_t)waves * max_bins * sizeof(float) + 2 * waves * kSnrBwdMaxW * 4;
+    hipLaunchKernelGGL(snr_backward_pairs_kernel, dim3(nblk), dim3(64 * waves), lds, s, pairs, blk,
+                       widths, nw);
+    return hipGetLastError();
+}
+
+hipError_t launch_ffa_adjoint_level_pairs(const GradPair* pairs, const int4* blk, int nblk,
+                                          hipStream_t s)
+{
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ffa_adjoint_level_pairs_kernel, dim3(nblk), dim3(256), 0, s, pairs, blk);
+    return hipGetLastError();
+}
+
+hipError_t launch_fold_groups(const GradGroup* groups, const GradPair* pairs,
+                              const GradStep* steps, const int4* blk, int nblk, hipStream_t s)
+{
+    if (nblk <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fold_groups_kernel, dim3(nblk), dim3(256), 0, s, groups, pairs, steps, blk);
+    return hipGetLastError();
+}
+
+hipError_t launch_ds_adjoint_series(const GradGroup* groups, const GradStep* steps, const int* g0,
+                                    int nseries, int64_t N, float* gz, hipStream_t s)
+{
+    if (nseries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(ds_adjoint_series_kernel, dim3(grid_of(N, 256), nseries), dim3(256), 0, s,
+                       groups, steps, g0, N, gz);
+    return hipGetLastError();
+}
+
+hipError_t launch_normalise_adjoint_batch(const float* g, const float* y, int64_t N, int nseries,
+                                          const float* inv_std, float* part, int nparts,
+                                          float* gx, hipStream_t s)
+{
+    if (nseries <= 0) return hipSuccess;
+    hipLaunchKernelGGL(normalise_adjoint_partial_batch_kernel, dim3(nparts, nseries), dim3(256), 0,
+                       s, g, y, N, part, nparts);
+    hipLaunchKernelGGL(normalise_adjoint_apply_batch_kernel, dim3(grid_of(N, 256), nseries),
+                       dim3(256), 0, s, g, y, N, (const float*)part, nparts, inv_std, gx);
+    return hipGetLastError();
+}
+
 
 hipError_t launch_snr_backward(const float* x, int rows, int bins, const float* gsnr,
                                const int32_t* widths, const float* hb, int nw, float stdnoise,

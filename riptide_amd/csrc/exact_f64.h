@@ -218,4 +218,18 @@ __host__ __device__ inline int64_t ds_imin(int64_t k, uint64_t mf, int ef)
     return (int64_t)((hi << (64 - F)) | (lo >> F));
 }
 
+
+#ifdef __HIPCC__
+// Sum of in[imin..imax] with the reference's edge weights (downsample.hpp:
+// 73-78); the interior is accumulated left to right and the last term fused
+// as the reference build does (acc + wmax*x -> fma).
+__device__ __forceinline__ float ds_accumulate(const float* in, const DsIndex& ix)
+{
+    float acc = ix.wmin * in[ix.imin];
+    for (int64_t i = ix.imin + 1; i < ix.imax; ++i)
+        acc += in[i];
+    return __fmaf_rn(ix.wmax, in[ix.imax], acc);
+}
+#endif
+
 }  // namespace rt

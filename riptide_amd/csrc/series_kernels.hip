@@ -16,17 +16,6 @@ namespace rt {
 // Downsampling (downsample.hpp:44-82)
 // ---------------------------------------------------------------------------
 
-// Sum of in[imin..imax] with the reference's edge weights; the interior is
-// accumulated left to right and the last term fused as the reference build
-// does (acc + wmax*x -> fma).
-__device__ __forceinline__ float ds_accumulate(const float* in, const DsIndex& ix)
-{
-    float acc = ix.wmin * in[ix.imin];
-    for (int64_t i = ix.imin + 1; i < ix.imax; ++i)
-        acc += in[i];
-    return __fmaf_rn(ix.wmax, in[ix.imax], acc);
-}
-
 // One thread per output sample, inputs read straight from HBM.  Used for
 // factors too large for the LDS-staged kernel.
 __global__ void downsample_direct_kernel(const float* __restrict__ in, int64_t N, uint64_t mf,

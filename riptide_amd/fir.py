@@ -134,38 +134,77 @@ def whitening_objective(snrs, rows):
     return (best - med).sum()
 
 
-def optimise_whitening(xs, tsamp, periods, steps=50, ntaps=129, lr=2e-3, period_min=0.5,
-                       period_max=5.0, bins_min=240, bins_max=260, log=None):
-    """Optimises one FIR filter per series of xs (S x N float32 CUDA tensor)
-    by `steps` Adam steps; periods[s] is the period whose S/N series s should
-    maximise.  Returns (taps, objective history: steps+1 arrays of S values)."""
-    from .autograd import periodogram_fn, periodogram_grid
-    from .ffautils import generate_width_trials
+def _row_index(periods_grid, periods, frac=0.005):
+    """(S, R) long tensor of the trial-period rows within +-frac of each
+    series' period, padded with each row list's first entry."""
     torch = _torch()
-    S, N = xs.shape
-    widths = generate_width_trials(bins_min)
-    grid, _ = periodogram_grid(N, tsamp, period_min, period_max, bins_min, bins_max)
-    rows = [near_period_rows(grid, p) for p in periods]
+    rows = [near_period_rows(periods_grid, p, frac) for p in periods]
     if any(len(r) == 0 for r in rows):
         raise ValueError("every target period must lie inside the searched period range")
+    R = max(len(r) for r in rows)
+    idx = np.stack([np.concatenate([r, np.full(R - len(r), r[0])]) for r in rows])
+    return torch.as_tensor(idx, dtype=torch.long, device=_lib.device())
+
+
+def batch_whitening_objective(snrs, row_index):
+    """Per series: sum over widths of (max S/N over its rows) - (median S/N
+    over all periods); snrs is (S, periods, widths), row_index (S, R)."""
+    torch = _torch()
+    idx = row_index[:, :, None].expand(-1, -1, snrs.shape[2])
+    best = torch.gather(snrs, 1, idx).max(dim=1).values
+    med = snrs.median(dim=1).values
+    return (best - med).sum(dim=1)
+
+
+def optimise_whitening(xs, tsamp, periods, steps=50, ntaps=129, lr=2e-3, period_min=0.5,
+                       period_max=5.0, bins_min=240, bins_max=260, log=None, batch=None):
+    """Optimises one FIR filter per series of xs (S x N float32 CUDA tensor)
+    by `steps` Adam steps; periods[s] is the period whose S/N series s should
+    maximise.  The whole batch runs through one BatchPeriodogram (batched
+    forward and adjoint).  Returns (taps, objective history: steps+1 arrays of
+    S values)."""
+    from .autograd import BatchPeriodogram
+    S, N = xs.shape
+    bp = batch or BatchPeriodogram(S, N, tsamp, None, period_min, period_max, bins_min,
+                                   bins_max)
+    grid, _ = bp.grid()
+    rows = _row_index(grid, periods)
     taps = identity_taps(S, ntaps, xs.device)
     opt = Adam(taps, lr=lr)
     history = []
-    for it in range(steps + 1):
-        t = taps.detach().requires_grad_(True)
-        y = fir_filter(xs, t)
-        objs = []
-        total = 0.0
-        for s in range(S):
-            snrs = periodogram_fn(y[s], tsamp, widths, period_min, period_max, bins_min, bins_max)
-            o = whitening_objective(snrs, rows[s])
-            objs.append(o.detach())
-            total = total + o
-        history.append(torch.stack(objs).cpu().numpy())
-        if log:
-            log(f"step {it}: mean objective {history[-1].mean():.3f}")
-        if it == steps:
-            break
-        (-total).backward()
-        opt.step(t.grad)
+    try:
+        for it in range(steps + 1):
+            t = taps.detach().requires_grad_(True)
+            objs = batch_whitening_objective(bp(fir_filter(xs, t)), rows)
+            history.append(objs.detach().cpu().numpy())
+            if log:
+                log(f"step {it}: mean objective {history[-1].mean():.3f}")
+            if it == steps:
+                break
+            (-objs.sum()).backward()
+            opt.step(t.grad)
+    finally:
+        if batch is None:
+            bp.close()
     return taps, history
+
+
+def detection_significance(snrs, periods_grid, period, exclude=0.02, harmonics=4):
+    """Significance of a pulsar at `period` in one periodogram (snrs:
+    periods x widths, numpy or tensor): for every width, the best S/N within
+    +-0.5% of the period, minus the median, divided by the robust standard
+    deviation ((p75 - p25) / 1.349) of the S/N at the trial periods away from
+    the period and its harmonics (k*P and P/k for k <= harmonics, +-exclude);
+    the largest over widths."""
+    S = np.asarray(snrs.detach().cpu().numpy() if hasattr(snrs, "detach") else snrs,
+                   dtype=np.float64)
+    P = np.asarray(periods_grid)
+    near = np.abs(P / period - 1) <= 0.005
+    off = np.ones(P.size, bool)
+    for k in range(1, harmonics + 1):
+        for q in (period * k, period / k):
+            off &= np.abs(P / q - 1) > exclude
+    best = S[near].max(axis=0)
+    p25, med, p75 = np.percentile(S[off], [25, 50, 75], axis=0)
+    z = (best - med) / ((p75 - p25) / 1.349)
+    return float(z.max())

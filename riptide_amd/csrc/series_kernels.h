@@ -6,12 +6,13 @@
 
 namespace rt {
 
-constexpr int kDsThreads = 256;
+constexpr int kDsThreads = 512;
 constexpr int kDsMaxSteps = 256;
 // Input samples staged per workgroup by the packed downsampling kernel, and
-// the largest factor it handles (its halo must fit next to the chunk in
-// 48 KiB of LDS); larger factors use the direct kernel.
-constexpr int kDsChunk = 8192;
+// the largest factor it handles (its halo, staged next to the chunk with
+// their two prefix arrays, must fit in LDS); larger factors use the direct
+// kernel.
+constexpr int kDsChunk = 4096;
 constexpr double kDsPackedMaxF = 4000.0;
 
 // One downsampling step: f = mf * 2^ef exactly (the float64 factor split

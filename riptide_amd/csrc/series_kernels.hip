@@ -93,14 +93,27 @@ __device__ __forceinline__ int64_t ds_first_at(int64_t c, const DsStepDev& st)
 // All downsampling steps of a search in one pass over the raw series: every
 // workgroup stages one chunk of the input (plus a halo of the largest factor)
 // in LDS and produces, for every step, the outputs whose first input sample
-// lies in the chunk.
+// lies in the chunk.  The staged samples also get an exclusive float-float
+// prefix sum, so that an output of a large factor costs its two edge terms
+// and one prefix difference (the interior sum to within about one rounding)
+// instead of a chain of f dependent adds; factors up to kDsDirectMax sum
+// their few interior samples directly, left to right.  Thread t takes the
+// flat outputs t, t + blockDim, ... of the chunk in step order, following
+// them with a step index that only moves forward.
+constexpr float kDsDirectMax = 12.0f;
+
 __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
     const DsSeriesDev* __restrict__ series, int nseries, const DsStepDev* __restrict__ steps,
     int chunk, int halo)
 {
-    extern __shared__ float s_in[];
+    extern __shared__ float s_mem[];
     __shared__ int64_t s_kbeg[kDsMaxSteps];
     __shared__ int s_cnt_prefix[kDsMaxSteps + 1];
+    __shared__ float s_th[kDsThreads], s_tl[kDsThreads];
+    const int cap = chunk + halo;
+    float* s_in = s_mem;             // staged samples
+    float* s_ch = s_in + cap + 1;    // exclusive prefix, high parts (cap + 1)
+    float* s_cl = s_ch + cap + 1;    // low parts
 
     // find the series of this chunk
     int lo = 0, hi = nseries - 1;
@@ -113,38 +126,108 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
     const DsSeriesDev sr = series[lo];
     const int64_t c0 = (int64_t)(b - sr.chunk_offset) * chunk;
     const int64_t c1 = min(c0 + chunk, sr.N);
-    const int64_t l1 = min(c0 + chunk + halo, sr.N);
-    for (int64_t i = c0 + threadIdx.x; i < l1; i += blockDim.x)
-        s_in[i - c0] = sr.x[i];
+    const int L = (int)(min(c0 + chunk + halo, sr.N) - c0);
+    const int tid = threadIdx.x;
+    for (int j = tid; j < L; j += blockDim.x) s_in[j] = sr.x[c0 + j];
 
     const DsStepDev* st = steps + sr.step0;
-    for (int s = threadIdx.x; s < sr.nsteps; s += blockDim.x) {
+    for (int s = tid; s < sr.nsteps; s += blockDim.x) {
         const int64_t kb = ds_first_at(c0, st[s]);
         const int64_t ke = (c1 >= sr.N) ? st[s].n : ds_first_at(c1, st[s]);
         s_kbeg[s] = kb;
         s_cnt_prefix[s + 1] = (int)(ke - kb);
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    // float-float running sum of this thread's segment of the staged samples
+    const int seg = (L + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int j0 = min(L, tid * seg), j1 = min(L, j0 + seg);
+    float sh = 0.0f, sl = 0.0f;
+    for (int j = j0; j < j1; ++j) {
+        float t, e;
+        two_sum(sh, s_in[j], t, e);
+        sh = t;
+        sl += e;
+    }
+    s_th[tid] = sh;
+    s_tl[tid] = sl;
+    if (tid == 0) {
         s_cnt_prefix[0] = 0;
         for (int s = 0; s < sr.nsteps; ++s)
             s_cnt_prefix[s + 1] += s_cnt_prefix[s];
     }
     __syncthreads();
-    const int total = s_cnt_prefix[sr.nsteps];
-    for (int e = threadIdx.x; e < total; e += blockDim.x) {
-        int a = 0, z = sr.nsteps - 1;
-        while (a < z) {
-            const int mid = (a + z + 1) >> 1;
-            if (s_cnt_prefix[mid] <= e) a = mid;
-            else z = mid - 1;
+    // exclusive scan of the thread totals by the first wavefront: lane l
+    // takes totals Q*l .. Q*l + Q - 1
+    constexpr int Q = kDsThreads / kWave;
+    static_assert(Q * kWave == kDsThreads, "threads must be whole wavefronts");
+    if (tid < kWave) {
+        float ph[Q], pl[Q];
+        float ah = 0.0f, al = 0.0f;
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            ph[q] = ah;
+            pl[q] = al;
+            ff_add(ah, al, s_th[Q * tid + q], s_tl[Q * tid + q], ah, al);
         }
+        float ih = ah, il = al;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const float oh = __shfl_up(ih, d, kWave);
+            const float ol = __shfl_up(il, d, kWave);
+            if (tid >= d) ff_add(oh, ol, ih, il, ih, il);
+        }
+        float eh = __shfl_up(ih, 1, kWave);
+        float el = __shfl_up(il, 1, kWave);
+        if (tid == 0) {
+            eh = 0.0f;
+            el = 0.0f;
+        }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            float rh, rl;
+            ff_add(eh, el, ph[q], pl[q], rh, rl);
+            s_th[Q * tid + q] = rh;
+            s_tl[Q * tid + q] = rl;
+        }
+    }
+    __syncthreads();
+    // the segment rescanned from its offset: C[j] = sum of s_in[0 .. j)
+    sh = s_th[tid];
+    sl = s_tl[tid];
+    for (int j = j0; j < j1; ++j) {
+        s_ch[j] = sh + sl;
+        s_cl[j] = sl - ((sh + sl) - sh);
+        float t, e;
+        two_sum(sh, s_in[j], t, e);
+        sh = t;
+        sl += e;
+    }
+    if (j1 == L && j0 < j1) {
+        s_ch[L] = sh + sl;
+        s_cl[L] = sl - ((sh + sl) - sh);
+    }
+    __syncthreads();
+
+    const int total = s_cnt_prefix[sr.nsteps];
+    int a = 0;
+    for (int e = tid; e < total; e += blockDim.x) {
+        while (s_cnt_prefix[a + 1] <= e) ++a;
         const DsStepDev& p = st[a];
         const int64_t k = s_kbeg[a] + (e - s_cnt_prefix[a]);
         DsIndex ix = ds_index_fast(k, p.mf, p.ef, sr.N);
         ix.imin -= c0;
         ix.imax -= c0;
-        sr.out[p.out_offset + k] = ds_accumulate(s_in, ix);
+        float v;
+        if (p.inv_f >= 1.0f / kDsDirectMax) {
+            v = ds_accumulate(s_in, ix);
+        } else {
+            const int ia = (int)ix.imin, ib = (int)ix.imax;
+            float inner = 0.0f;
+            if (ib > ia + 1)
+                inner = (s_ch[ib] - s_ch[ia + 1]) + (s_cl[ib] - s_cl[ia + 1]);
+            v = __fmaf_rn(ix.wmax, s_in[ib], ix.wmin * s_in[ia] + inner);
+        }
+        sr.out[p.out_offset + k] = v;
     }
 }
 
@@ -708,7 +791,16 @@ hipError_t launch_downsample_packed(const DsSeriesDev* d_series, int nseries,
                                     const DsStepDev* d_steps, int total_chunks, int chunk,
                                     int halo, hipStream_t stream)
 {
-    const size_t lds = (size_t)(chunk + halo) * sizeof(float);
+    // staged samples and the two prefix arrays
+    const size_t lds = (size_t)3 * (chunk + halo + 1) * sizeof(float);
+    static size_t configured = 0;
+    if (lds > 48 * 1024 && lds > configured) {
+        const hipError_t e = hipFuncSetAttribute((const void*)downsample_packed_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return e;
+        configured = lds;
+    }
     hipLaunchKernelGGL(downsample_packed_kernel, dim3(total_chunks), dim3(kDsThreads), lds, stream,
                        d_series, nseries, d_steps, chunk, halo);
     return hipGetLastError();

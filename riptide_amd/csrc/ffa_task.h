@@ -639,10 +639,10 @@ __device__ __forceinline__ void window_max_default(const float* cps, int i0, int
 //     into C chunks; one thread per (chunk, row), chunk-major so that the
 //     lanes of a wavefront take the same chunk of consecutive rows, with C
 //     chosen so that about every thread of the workgroup has an item.
-//     Chunk totals in float-float go to LDS, each row's totals are scanned,
-//     then every chunk is rescanned from the float-float sum of the chunks
-//     before it, writing each partial sum rounded once to float32 -- what
-//     the reference's float64 accumulator gives.
+//     Chunk totals in float-float go to LDS, then every item rescans its
+//     chunk from the float-float sum of the chunks before it (which it adds
+//     up itself, in chunk order), writing each partial sum rounded once to
+//     float32 -- what the reference's float64 accumulator gives.
 //  2. window maxima: the same (chunk, row) threads, all widths per thread
 //     (each prefix value read once per width), combined across chunks with
 //     LDS integer max; then one thread per (row, width) writes the S/N.
@@ -714,28 +714,20 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         if (lane < 2 * nwl) s_hb[lane] = hb_ld;
     }
     __syncthreads();
-    // (2) exclusive scan of each row's chunk totals, in place
-    for (int p = tid; p < nrows; p += NT) {
-        float oh = 0.0f, ol = 0.0f;
-        for (int c = 0; c < C; ++c) {
-            const float th = tot[2 * (c * nrows + p)];
-            const float tl = tot[2 * (c * nrows + p) + 1];
-            tot[2 * (c * nrows + p)] = oh;
-            tot[2 * (c * nrows + p) + 1] = ol;
-            ff_add(oh, ol, th, tl, oh, ol);
-        }
-    }
-    __syncthreads();
-    // (3) every chunk rescanned from the sum of the chunks before it, each
-    // partial sum rounded once to float32 -- what the reference's float64
-    // accumulator gives
+    // (2) every chunk rescanned from the sum of the chunks before it (added
+    // in chunk order by the item itself), each partial sum rounded once to
+    // float32 -- what the reference's float64 accumulator gives.  The chunk
+    // totals stay in place: items only read them, so no barrier is needed
+    // before the rows are overwritten.
     for (int it = tid; it < nitems; it += NT) {
         const int c = it / nrows;
         const int p = it - c * nrows;
+        float sh = 0.0f, sl = 0.0f;
+        for (int q = 0; q < c; ++q)
+            ff_add(sh, sl, tot[2 * (q * nrows + p)], tot[2 * (q * nrows + p) + 1], sh, sl);
         float* row = data + (size_t)p * pitch;
         int j = 4 * ((c * g4) / C);
         const int je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
-        float sh = tot[2 * it], sl = tot[2 * it + 1];
         for (; j + 4 <= je; j += 4) {
             float x[4];
 #pragma unroll

@@ -218,6 +218,27 @@ __host__ __device__ inline int64_t ds_imin(int64_t k, uint64_t mf, int ef)
     return (int64_t)((hi << (64 - F)) | (lo >> F));
 }
 
+// The same index math in native float64 arithmetic, for k and N below 2^31:
+// the float64 product and sum are each rounded once, exactly as the
+// reference's doubles and the integer emulation above (the two agree bit for
+// bit), at a few instructions per output.  Build with -ffp-contract=off so
+// that k*f + f is not fused.  f = mf * 2^ef is exact in a double.
+__host__ __device__ inline double ds_factor(uint64_t mf, int ef) { return ldexp((double)mf, ef); }
+
+__host__ __device__ inline DsIndex ds_index_f64(int k, double f, int N)
+{
+    const double start = (double)k * f;
+    const double end = start + f;
+    const int imin = (int)start;  // start >= 0: truncation is the floor
+    const int ie = (int)end;
+    const int imax = ie < N - 1 ? ie : N - 1;
+    DsIndex r;
+    r.imin = imin;
+    r.imax = imax;
+    r.wmin = (float)((double)(imin + 1) - start);
+    r.wmax = (float)(end - (double)imax);
+    return r;
+}
 
 #ifdef __HIPCC__
 // Sum of in[imin..imax] with the reference's edge weights (downsample.hpp:

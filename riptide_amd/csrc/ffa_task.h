@@ -765,7 +765,10 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
 #pragma unroll
         for (int j = 0; j < kSnrMaxWidths; ++j)
             w[j] = __builtin_amdgcn_readfirstlane(s_w[j < nw ? j : nw - 1]);
-        const int wmax = __builtin_amdgcn_readfirstlane(s_w[nw - 1]);
+        // the widest window (the widths need not be in order)
+        int wmax = w[0];
+#pragma unroll
+        for (int j = 1; j < kSnrMaxWidths; ++j) wmax = max(wmax, w[j]);
         // default width trials (7 or more): the compile-time path
         bool dflt = nw >= 7;
 #pragma unroll
@@ -809,9 +812,13 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             for (int j = 0; j < kSnrMaxWidths; ++j)
                 if (j < nw) atomicMax(&mx[p * nw + j], ordered_int(m[j]));
         }
-        // (b) the last wmax window starts, one thread per (row, width),
-        // width-major: windows inside the row, then the ones running past its
-        // end over the extension cps[j] + total
+        // (b) the last wmax window starts [P, bins), one thread per (row,
+        // width), width-major.  The window at k ends inside the row while k <
+        // split = bins - wj, else on the extension cps[j] + total (kernels.hpp:
+        // 88-100).  Eight starts per step with all sixteen loads issued
+        // before the first use; the step that crosses split and the last
+        // partial step take the per-start form (starts past the row clamped
+        // to its last start, which leaves the maximum unchanged).
         for (int job = tid; job < nrows * nw; job += NT) {
             const int iw = job / nrows;
             const int p = job - iw * nrows;
@@ -819,32 +826,53 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             const int wj = s_w[iw];
             const float total = cps[bins - 1];
             const int split = bins - wj;
-            float d0 = -INFINITY, d1 = -INFINITY;
+            float d = -INFINITY;
             int k = P;
-            for (; k + 4 <= split; k += 4) {
-                float a[4], b[4];
+            auto mixed = [&](int k0) {
+                float a[8], b[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < 8; ++u) {
+                    const int st = min(k0 + u, bins - 1);
+                    const int e = st + wj;
+                    a[u] = cps[st];
+                    b[u] = cps[e < bins ? e : e - bins];
+                }
+                asm volatile("" ::: "memory");
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int st = min(k0 + u, bins - 1);
+                    const float v = st < split ? b[u] : b[u] + total;
+                    d = fmaxf(d, v - a[u]);
+                }
+            };
+            for (; k + 8 <= split; k += 8) {
+                float a[8], b[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
                     a[u] = cps[k + u];
                     b[u] = cps[k + u + wj];
                 }
                 asm volatile("" ::: "memory");
-                d0 = fmaxf(d0, fmaxf(fmaxf(b[0] - a[0], b[1] - a[1]), fmaxf(b[2] - a[2], b[3] - a[3])));
+                d = fmaxf(d, fmaxf(fmaxf(fmaxf(b[0] - a[0], b[1] - a[1]), fmaxf(b[2] - a[2], b[3] - a[3])),
+                                   fmaxf(fmaxf(b[4] - a[4], b[5] - a[5]), fmaxf(b[6] - a[6], b[7] - a[7]))));
             }
-            for (; k < split; ++k) d0 = fmaxf(d0, cps[k + wj] - cps[k]);
-            for (; k + 4 <= bins; k += 4) {
-                float a[4], b[4];
+            if (k < split) {
+                mixed(k);
+                k += 8;
+            }
+            for (; k + 8 <= bins; k += 8) {
+                float a[8], b[8];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < 8; ++u) {
                     a[u] = cps[k + u];
                     b[u] = cps[k + u - split];
                 }
                 asm volatile("" ::: "memory");
 #pragma unroll
-                for (int u = 0; u < 4; ++u) d1 = fmaxf(d1, (b[u] + total) - a[u]);
+                for (int u = 0; u < 8; ++u) d = fmaxf(d, (b[u] + total) - a[u]);
             }
-            for (; k < bins; ++k) d1 = fmaxf(d1, (cps[k - split] + total) - cps[k]);
-            atomicMax(&mx[p * nw + iw], ordered_int(fmaxf(d0, d1)));
+            if (k < bins) mixed(k);
+            atomicMax(&mx[p * nw + iw], ordered_int(d));
         }
         __syncthreads();
         for (int e = tid; e < nrows * nw; e += NT) {

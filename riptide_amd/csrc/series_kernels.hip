@@ -110,6 +110,10 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
     __shared__ int64_t s_kbeg[kDsMaxSteps];
     __shared__ int s_cnt_prefix[kDsMaxSteps + 1];
     __shared__ float s_th[kDsThreads], s_tl[kDsThreads];
+    // per step: the factor as a double, its reciprocal and the output row
+    __shared__ double s_f[kDsMaxSteps];
+    __shared__ float s_invf[kDsMaxSteps];
+    __shared__ float* s_out[kDsMaxSteps];
     const int cap = chunk + halo;
     float* s_in = s_mem;             // staged samples
     float* s_ch = s_in + cap + 1;    // exclusive prefix, high parts (cap + 1)
@@ -136,6 +140,9 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
         const int64_t ke = (c1 >= sr.N) ? st[s].n : ds_first_at(c1, st[s]);
         s_kbeg[s] = kb;
         s_cnt_prefix[s + 1] = (int)(ke - kb);
+        s_f[s] = ds_factor(st[s].mf, st[s].ef);
+        s_invf[s] = st[s].inv_f;
+        s_out[s] = sr.out + st[s].out_offset;
     }
     __syncthreads();
     // float-float running sum of this thread's segment of the staged samples
@@ -209,16 +216,18 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
     __syncthreads();
 
     const int total = s_cnt_prefix[sr.nsteps];
+    // series below 2^31 samples: the index math in native float64
+    const bool f64 = sr.N < ((int64_t)1 << 31);
     int a = 0;
     for (int e = tid; e < total; e += blockDim.x) {
         while (s_cnt_prefix[a + 1] <= e) ++a;
-        const DsStepDev& p = st[a];
         const int64_t k = s_kbeg[a] + (e - s_cnt_prefix[a]);
-        DsIndex ix = ds_index_fast(k, p.mf, p.ef, sr.N);
+        DsIndex ix = f64 ? ds_index_f64((int)k, s_f[a], (int)sr.N)
+                         : ds_index_fast(k, st[a].mf, st[a].ef, sr.N);
         ix.imin -= c0;
         ix.imax -= c0;
         float v;
-        if (p.inv_f >= 1.0f / kDsDirectMax) {
+        if (s_invf[a] >= 1.0f / kDsDirectMax) {
             v = ds_accumulate(s_in, ix);
         } else {
             const int ia = (int)ix.imin, ib = (int)ix.imax;
@@ -227,7 +236,7 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
                 inner = (s_ch[ib] - s_ch[ia + 1]) + (s_cl[ib] - s_cl[ia + 1]);
             v = __fmaf_rn(ix.wmax, s_in[ib], ix.wmin * s_in[ia] + inner);
         }
-        sr.out[p.out_offset + k] = v;
+        s_out[a][k] = v;
     }
 }
 
@@ -385,57 +394,120 @@ __global__ void scrunch_mean_kernel(const float* __restrict__ x, int64_t nblocks
                        (int64_t)gridDim.x * blockDim.x);
 }
 
+// Samples of the elementwise preparation kernels are taken four at a time
+// (group g = samples 4g .. 4g + 3): one 16-byte access per lane when both
+// buffers are 16-byte aligned, scalar accesses otherwise, the same arithmetic
+// either way.
+__device__ __forceinline__ int load_group(const float* __restrict__ x, int64_t i0, int64_t N,
+                                          bool vec, float (&v)[4])
+{
+    const int cnt = (int)min((int64_t)4, N - i0);
+    if (vec && cnt == 4) {
+        const float4 t = *reinterpret_cast<const float4*>(x + i0);
+        v[0] = t.x;
+        v[1] = t.y;
+        v[2] = t.z;
+        v[3] = t.w;
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = u < cnt ? x[i0 + u] : 0.0f;
+    }
+    return cnt;
+}
+
+__device__ __forceinline__ void store_group(float* __restrict__ out, int64_t i0, int cnt, bool vec,
+                                            const float (&v)[4])
+{
+    if (vec && cnt == 4) {
+        *reinterpret_cast<float4*>(out + i0) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (u < cnt) out[i0 + u] = v[u];
+    }
+}
+
+__device__ __forceinline__ bool aligned16(const void* a, const void* b)
+{
+    return ((reinterpret_cast<uintptr_t>(a) | reinterpret_cast<uintptr_t>(b)) & 15) == 0;
+}
+
+// numpy's linear interpolation of the low-resolution running median fp at
+// sample i, as a float-float (vh, vl): num = 2i - sf + 1 and j the last node
+// with 2*x_j = 2*j*sf + sf - 1 <= 2i (-1 before the first), clamped to the
+// end values, exact node values on the nodes.
+__device__ __forceinline__ void rmed_interp(const float* __restrict__ fp, int64_t nl, int64_t sf,
+                                            int64_t num, int64_t j, float& vh, float& vl)
+{
+    if (j < 0) {
+        vh = fp[0];
+        vl = 0.0f;
+    } else if (j >= nl - 1) {
+        vh = fp[nl - 1];
+        vl = 0.0f;
+    } else {
+        const int64_t t2 = num - 2 * j * sf;  // 2*(i - x_j), in [0, 2*sf)
+        if (t2 == 0) {
+            vh = fp[j];
+            vl = 0.0f;
+        } else {
+            // (fp[j+1] - fp[j]) * (t2/2) / sf + fp[j]
+            float dh, dl;
+            two_sum(fp[j + 1], -fp[j], dh, dl);
+            const float t = 0.5f * (float)t2;  // exact (t2 < 2^24)
+            const float ph = dh * t;
+            const float pl = __fmaf_rn(dh, t, -ph) + dl * t;
+            const float fsf = (float)sf;
+            const float qh = ph / fsf;
+            const float r = __fmaf_rn(-qh, fsf, ph) + pl;
+            const float ql = r / fsf;
+            float sh, se;
+            two_sum(fp[j], qh, sh, se);
+            vh = sh;
+            vl = se + ql;
+        }
+    }
+}
+
 // out[i] = float32(x[i] - interp(i)) (or float32(interp(i)) when x is null),
-// where interp is numpy's linear
-// interpolation of the low-resolution running median at centres
-// j*sf + (sf-1)/2 (np.interp semantics: clamp to the end values, exact node
-// values on the nodes).  Evaluated in float-float arithmetic, rounded once.
+// where interp is numpy's linear interpolation of the low-resolution running
+// median at centres j*sf + (sf-1)/2 (np.interp semantics), evaluated in
+// float-float arithmetic and rounded once.  Groups of four samples from
+// `first`, `stride` groups apart; one integer division per group (the node
+// index moves by at most one per sample).
 __device__ __forceinline__ void deredden_subtract_range(const float* __restrict__ x, int64_t N,
                                                         const float* __restrict__ fp, int64_t nl,
                                                         int64_t sf, float* __restrict__ out,
                                                         int64_t first, int64_t stride)
 {
-    for (int64_t i = first; i < N; i += stride) {
-        // 2*x_j = 2*j*sf + sf - 1; node j is the last with 2*x_j <= 2*i
-        const int64_t num = 2 * i - sf + 1;
-        int64_t j = (num >= 0) ? num / (2 * sf) : -1;
-        float vh, vl;
-        if (j < 0) {
-            vh = fp[0];
-            vl = 0.0f;
-        } else if (j >= nl - 1) {
-            vh = fp[nl - 1];
-            vl = 0.0f;
-        } else {
-            const int64_t t2 = num - 2 * j * sf;  // 2*(i - x_j), in [0, 2*sf)
-            if (t2 == 0) {
-                vh = fp[j];
-                vl = 0.0f;
+    const bool vec = aligned16(x, out);
+    const int64_t two_sf = 2 * sf;
+    for (int64_t g = first; 4 * g < N; g += stride) {
+        const int64_t i0 = 4 * g;
+        float xv[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+        int cnt = (int)min((int64_t)4, N - i0);
+        if (x != nullptr) cnt = load_group(x, i0, N, vec, xv);
+        int64_t num = 2 * i0 - sf + 1;
+        int64_t j = (num >= 0) ? num / two_sf : -1;
+        float r[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u > 0) {
+                num += 2;
+                if (num >= two_sf * (j + 1)) ++j;
+            }
+            float vh, vl;
+            rmed_interp(fp, nl, sf, num, j, vh, vl);
+            if (x == nullptr) {
+                r[u] = vh + vl;  // the interpolated running median alone
             } else {
-                // (fp[j+1] - fp[j]) * (t2/2) / sf + fp[j]
-                float dh, dl;
-                two_sum(fp[j + 1], -fp[j], dh, dl);
-                const float t = 0.5f * (float)t2;  // exact (t2 < 2^24)
-                const float ph = dh * t;
-                const float pl = __fmaf_rn(dh, t, -ph) + dl * t;
-                const float fsf = (float)sf;
-                const float qh = ph / fsf;
-                const float r = __fmaf_rn(-qh, fsf, ph) + pl;
-                const float ql = r / fsf;
-                float sh, se;
-                two_sum(fp[j], qh, sh, se);
-                vh = sh;
-                vl = se + ql;
+                float d, e;
+                two_sum(xv[u], -vh, d, e);
+                e -= vl;
+                r[u] = d + e;
             }
         }
-        if (x == nullptr) {
-            out[i] = vh + vl;  // the interpolated running median alone
-            continue;
-        }
-        float d, e;
-        two_sum(x[i], -vh, d, e);
-        e -= vl;
-        out[i] = d + e;
+        store_group(out, i0, cnt, vec, r);
     }
 }
 
@@ -477,7 +549,8 @@ __device__ __forceinline__ void block_ff_reduce(float& h, float& l, float* s_h, 
 }
 
 // mode 0: partial sums of x; mode 1: partial sums of (x - mean)^2; part
-// `part` of `nparts` (256 threads each, grid-stride over the series)
+// `part` of `nparts` (256 threads each): thread t of part p takes the groups
+// of four samples p*256 + t, (nparts + p)*256 + t, ... in sample order
 __device__ __forceinline__ void moments_partial(const float* __restrict__ x, int64_t N, int mode,
                                                 const float* __restrict__ mean,
                                                 float* __restrict__ partial, int part, int nparts,
@@ -486,18 +559,24 @@ __device__ __forceinline__ void moments_partial(const float* __restrict__ x, int
     float h = 0.0f, l = 0.0f;
     const float mh = mode ? mean[0] : 0.0f;
     const float ml = mode ? mean[1] : 0.0f;
-    for (int64_t i = (int64_t)part * blockDim.x + threadIdx.x; i < N;
-         i += (int64_t)nparts * blockDim.x) {
-        const float v = x[i];
-        if (mode == 0) {
-            ff_add(h, l, v, 0.0f, h, l);
-        } else {
-            float dh, dl;
-            two_sum(v, -mh, dh, dl);
-            dl -= ml;
-            const float sh = dh * dh;
-            const float sl = __fmaf_rn(dh, dh, -sh) + 2.0f * dh * dl;
-            ff_add(h, l, sh, sl, h, l);
+    const bool vec = aligned16(x, x);
+    for (int64_t g = (int64_t)part * blockDim.x + threadIdx.x; 4 * g < N;
+         g += (int64_t)nparts * blockDim.x) {
+        float v[4];
+        const int cnt = load_group(x, 4 * g, N, vec, v);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (u >= cnt) break;
+            if (mode == 0) {
+                ff_add(h, l, v[u], 0.0f, h, l);
+            } else {
+                float dh, dl;
+                two_sum(v[u], -mh, dh, dl);
+                dl -= ml;
+                const float sh = dh * dh;
+                const float sl = __fmaf_rn(dh, dh, -sh) + 2.0f * dh * dl;
+                ff_add(h, l, sh, sl, h, l);
+            }
         }
     }
     block_ff_reduce(h, l, s_h, s_l);
@@ -562,7 +641,8 @@ __global__ __launch_bounds__(256) void moments_final_kernel(const float* __restr
     moments_final(partial, nparts, N, mode, result, s_h, s_l);
 }
 
-// out = float32((x - mean) / std) with float-float mean and std
+// out = float32((x - mean) / std) with float-float mean and std, groups of
+// four samples from `first`, `stride` groups apart
 __device__ __forceinline__ void normalise_apply_range(const float* __restrict__ x, int64_t N,
                                                       const float* __restrict__ mean,
                                                       const float* __restrict__ stdv,
@@ -571,16 +651,24 @@ __device__ __forceinline__ void normalise_apply_range(const float* __restrict__ 
 {
     const float mh = mean[0], ml = mean[1];
     const float sh = stdv[0], sl = stdv[1];
-    for (int64_t i = first; i < N; i += stride) {
-        float dh, dl;
-        two_sum(x[i], -mh, dh, dl);
-        dl -= ml;
-        // (dh + dl) / (sh + sl)
-        const float q = dh / sh;
-        const float p = q * sh;
-        const float pe = __fmaf_rn(q, sh, -p);
-        const float r = ((dh - p) - pe) + dl - q * sl;
-        out[i] = q + r / sh;
+    const bool vec = aligned16(x, out);
+    for (int64_t g = first; 4 * g < N; g += stride) {
+        const int64_t i0 = 4 * g;
+        float v[4];
+        const int cnt = load_group(x, i0, N, vec, v);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            float dh, dl;
+            two_sum(v[u], -mh, dh, dl);
+            dl -= ml;
+            // (dh + dl) / (sh + sl)
+            const float q = dh / sh;
+            const float p = q * sh;
+            const float pe = __fmaf_rn(q, sh, -p);
+            const float r = ((dh - p) - pe) + dl - q * sl;
+            v[u] = q + r / sh;
+        }
+        store_group(out, i0, cnt, vec, v);
     }
 }
 
@@ -829,7 +917,7 @@ hipError_t launch_scrunch_mean(const float* x, int64_t nblocks, int64_t factor, 
 hipError_t launch_deredden_subtract(const float* x, int64_t N, const float* fp, int64_t nl,
                                     int64_t sf, float* out, hipStream_t stream)
 {
-    hipLaunchKernelGGL(deredden_subtract_kernel, dim3(grid_for(N, 256)), dim3(256), 0, stream, x,
+    hipLaunchKernelGGL(deredden_subtract_kernel, dim3(grid_for((N + 3) / 4, 256)), dim3(256), 0, stream, x,
                        N, fp, nl, sf, out);
     return hipGetLastError();
 }
@@ -848,7 +936,7 @@ hipError_t launch_normalise(const float* x, int64_t N, float* out, float* work, 
                        (const float*)mean, partial);
     hipLaunchKernelGGL(moments_final_kernel, dim3(1), dim3(256), 0, stream, partial, parts, N, 1,
                        stdv);
-    hipLaunchKernelGGL(normalise_apply_kernel, dim3(grid_for(N, 256)), dim3(256), 0, stream, x, N,
+    hipLaunchKernelGGL(normalise_apply_kernel, dim3(grid_for((N + 3) / 4, 256)), dim3(256), 0, stream, x, N,
                        (const float*)mean, (const float*)stdv, out);
     return hipGetLastError();
 }
@@ -869,7 +957,7 @@ hipError_t launch_prep_batch(const PrepSeriesDev* d_ps, int nseries, int64_t max
         hipLaunchKernelGGL(prep_rmed_kernel,
                            dim3(grid_for(std::max(max_lores, (int64_t)1), 256, 4096), nseries),
                            dim3(256), use_lds ? span * sizeof(float) : 0, stream, d_ps, use_lds);
-        hipLaunchKernelGGL(prep_subtract_kernel, dim3(grid_for(max_n, 256, 2048), nseries), dim3(256),
+        hipLaunchKernelGGL(prep_subtract_kernel, dim3(grid_for((max_n + 3) / 4, 256, 2048), nseries), dim3(256),
                            0, stream, d_ps);
     }
     if (normalise) {
@@ -881,7 +969,7 @@ hipError_t launch_prep_batch(const PrepSeriesDev* d_ps, int nseries, int64_t max
                            0, stream, d_ps, 1);
         hipLaunchKernelGGL(prep_moments_final_kernel, dim3(1, nseries), dim3(256), 0, stream, d_ps,
                            1);
-        const int gx = grid_for(max_n, 256);
+        const int gx = grid_for((max_n + 3) / 4, 256);
         hipLaunchKernelGGL(prep_normalise_apply_kernel, dim3(gx, nseries), dim3(256), 0, stream,
                            d_ps, gx);
     }

@@ -181,6 +181,10 @@ SHAPE_CASES = [
     (2 ** 16, 1e-3, 0.05, 0.5, 40, 44, None),
     (2 ** 17, 256e-6, 0.5, 2.0, 240, 260, [1, 3, 5, 8, 21, 40]),
     (2 ** 17, 256e-6, 0.5, 2.0, 480, 520, [2, 7]),
+    # widths out of order, and windows up to nearly a whole row (most window
+    # starts then wrap around the row)
+    (2 ** 17, 256e-6, 0.5, 2.0, 240, 260, [13, 1, 40, 3, 7]),
+    (2 ** 16, 1e-3, 0.05, 0.5, 40, 44, [5, 39, 1, 17]),
 ]
 
 

@@ -90,6 +90,26 @@ __device__ __forceinline__ int64_t ds_first_at(int64_t c, const DsStepDev& st)
     return k;
 }
 
+// The reference's sum of staged samples in[ia..ib] (downsample.hpp:73-78):
+// wmin*in[ia], the interior samples left to right, then the last one fused.
+// The first NI interior samples are loaded together (a factor f has at most
+// floor(f) + 1 of them; more than NI take the loop at the end).
+template <int NI>
+__device__ __forceinline__ float ds_direct(const float* in, int ia, int ib, float wmin, float wmax)
+{
+    const int n = ib - ia - 1;  // interior samples (-1: ia == ib)
+    const int last = n > 1 ? n - 1 : 0;
+    float t[NI];
+#pragma unroll
+    for (int j = 0; j < NI; ++j) t[j] = in[ia + 1 + min(j, last)];
+    float acc = wmin * in[ia];
+#pragma unroll
+    for (int j = 0; j < NI; ++j)
+        if (j < n) acc += t[j];
+    for (int i = ia + 1 + NI; i < ib; ++i) acc += in[i];
+    return __fmaf_rn(wmax, in[ib], acc);
+}
+
 // All downsampling steps of a search in one pass over the raw series: every
 // workgroup stages one chunk of the input (plus a halo of the largest factor)
 // in LDS and produces, for every step, the outputs whose first input sample
@@ -218,72 +238,125 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
     const int total = s_cnt_prefix[sr.nsteps];
     // series below 2^31 samples: the index math in native float64
     const bool f64 = sr.N < ((int64_t)1 << 31);
-    int a = 0;
+    const int n32 = (int)min(sr.N, (int64_t)INT32_MAX);
+    // the step of the current output, its descriptor in registers, moved on
+    // when the flat output index crosses into a later step
+    int a = 0, a_first = 0, a_end = s_cnt_prefix[1];
+    int64_t kb = s_kbeg[0];
+    double f = s_f[0];
+    float invf = s_invf[0];
+    float* orow = s_out[0];
     for (int e = tid; e < total; e += blockDim.x) {
-        while (s_cnt_prefix[a + 1] <= e) ++a;
-        const int64_t k = s_kbeg[a] + (e - s_cnt_prefix[a]);
-        DsIndex ix = f64 ? ds_index_f64((int)k, s_f[a], (int)sr.N)
-                         : ds_index_fast(k, st[a].mf, st[a].ef, sr.N);
-        ix.imin -= c0;
-        ix.imax -= c0;
+        if (e >= a_end) {
+            do {
+                ++a;
+            } while (s_cnt_prefix[a + 1] <= e);
+            a_first = s_cnt_prefix[a];
+            a_end = s_cnt_prefix[a + 1];
+            kb = s_kbeg[a];
+            f = s_f[a];
+            invf = s_invf[a];
+            orow = s_out[a];
+        }
+        const int64_t k = kb + (e - a_first);
+        const DsIndex ix = f64 ? ds_index_f64((int)k, f, n32)
+                               : ds_index_fast(k, st[a].mf, st[a].ef, sr.N);
+        const int ia = (int)(ix.imin - c0), ib = (int)(ix.imax - c0);
         float v;
-        if (s_invf[a] >= 1.0f / kDsDirectMax) {
-            v = ds_accumulate(s_in, ix);
+        if (invf >= 1.0f / 4.0f) {
+            v = ds_direct<5>(s_in, ia, ib, ix.wmin, ix.wmax);
+        } else if (invf >= 1.0f / 8.0f) {
+            v = ds_direct<9>(s_in, ia, ib, ix.wmin, ix.wmax);
+        } else if (invf >= 1.0f / kDsDirectMax) {
+            v = ds_direct<13>(s_in, ia, ib, ix.wmin, ix.wmax);
         } else {
-            const int ia = (int)ix.imin, ib = (int)ix.imax;
             float inner = 0.0f;
             if (ib > ia + 1)
                 inner = (s_ch[ib] - s_ch[ia + 1]) + (s_cl[ib] - s_cl[ia + 1]);
             v = __fmaf_rn(ix.wmax, s_in[ib], ix.wmin * s_in[ia] + inner);
         }
-        s_out[a][k] = v;
+        orow[k] = v;
     }
 }
 
 // ---------------------------------------------------------------------------
 // Running median (running_median.hpp:100-132): exact order statistic of the
-// edge-padded window, found by counting (pure comparisons, bit-exact).
+// edge-padded window (pure comparisons, bit-exact).
 // ---------------------------------------------------------------------------
 
-// Median of the edge-padded window of `width` samples centred on output
-// o0 + threadIdx.x, counting ranks; `win` holds x[o0 - half .. o0 + blockDim.x
-// + half) (clamped) when use_lds, else x is read directly.
+// LDS floats of running_median_block's sorted path for `threads` outputs.
+__host__ __device__ inline size_t rmed_lds_floats(int threads, int width)
+{
+    return (size_t)3 * (threads + width - 1);
+}
+
+// Median of the edge-padded window of `width` (odd) samples centred on
+// output o0 + threadIdx.x.
+//  sorted (use_lds): the workgroup's union window x[o0 - half .. o0 +
+//    blockDim.x + half) (clamped) goes to LDS and every sample gets its
+//    stable rank (samples below it, plus equal ones before it) by one pass of
+//    comparisons; then each output walks the union in sorted order, counting
+//    the samples of its own window, and takes the one that completes half + 1.
+//    About 2 (blockDim + width) comparisons per output.
+//  counting (otherwise): each output tests the samples of its window in turn,
+//    counting those below and not above each candidate (about width^2 / 2
+//    comparisons), reading x directly.
 __device__ __forceinline__ void running_median_block(const float* __restrict__ x, int64_t N,
                                                      int width, int use_lds, int64_t o0,
                                                      float* s_win, float* __restrict__ out)
 {
     const int half = width / 2;
     const int span = blockDim.x + width - 1;
+    const int t = threadIdx.x;
+    const int64_t i = o0 + t;
     if (use_lds) {
-        for (int j = threadIdx.x; j < span; j += blockDim.x) {
+        float* s_val = s_win;                                  // window samples
+        float* s_sv = s_win + span;                            // sorted samples
+        int* s_si = reinterpret_cast<int*>(s_win + 2 * span);  // their window positions
+        for (int j = t; j < span; j += blockDim.x) {
             int64_t src = o0 + j - half;
             src = src < 0 ? 0 : (src > N - 1 ? N - 1 : src);
-            s_win[j] = x[src];
+            s_val[j] = x[src];
         }
         __syncthreads();
+        for (int u = t; u < span; u += blockDim.x) {
+            const float v = s_val[u];
+            int r = 0;
+            for (int q = 0; q < span; ++q) {
+                const float y = s_val[q];
+                r += (y < v) || (y == v && q < u);
+            }
+            s_sv[r] = v;
+            s_si[r] = u;
+        }
+        __syncthreads();
+        if (i >= N) return;
+        float med = 0.0f;
+        int cnt = 0;
+        bool found = false;
+        for (int r = 0; r < span; ++r) {
+            const bool in = (unsigned)(s_si[r] - t) < (unsigned)width;
+            cnt += in ? 1 : 0;
+            if (!found && in && cnt == half + 1) {
+                med = s_sv[r];
+                found = true;
+            }
+            if (__all(found)) break;
+        }
+        out[i] = med;
+        return;
     }
-    const int64_t i = o0 + threadIdx.x;
     if (i >= N) return;
     float med = 0.0f;
     for (int c = 0; c < width; ++c) {
-        float cand;
-        if (use_lds) {
-            cand = s_win[threadIdx.x + c];
-        } else {
-            int64_t src = i + c - half;
-            src = src < 0 ? 0 : (src > N - 1 ? N - 1 : src);
-            cand = x[src];
-        }
+        int64_t src = i + c - half;
+        src = src < 0 ? 0 : (src > N - 1 ? N - 1 : src);
+        const float cand = x[src];
         int lt = 0, le = 0;
         for (int j = 0; j < width; ++j) {
-            float v;
-            if (use_lds) {
-                v = s_win[threadIdx.x + j];
-            } else {
-                int64_t src = i + j - half;
-                src = src < 0 ? 0 : (src > N - 1 ? N - 1 : src);
-                v = x[src];
-            }
+            int64_t sj = i + j - half;
+            sj = sj < 0 ? 0 : (sj > N - 1 ? N - 1 : sj);
+            const float v = x[sj];
             lt += (v < cand);
             le += (v <= cand);
         }
@@ -898,11 +971,10 @@ hipError_t launch_running_median(const float* x, int64_t N, int width, float* ou
                                  hipStream_t stream)
 {
     const int threads = 256;
-    const size_t span = (size_t)threads + width - 1;
-    const int use_lds = span * sizeof(float) <= 48 * 1024;
+    const size_t lds = rmed_lds_floats(threads, width) * sizeof(float);
+    const int use_lds = lds <= 48 * 1024;
     hipLaunchKernelGGL(running_median_kernel, dim3((unsigned)((N + threads - 1) / threads)),
-                       dim3(threads), use_lds ? span * sizeof(float) : 0, stream, x, N, width,
-                       use_lds, out);
+                       dim3(threads), use_lds ? lds : 0, stream, x, N, width, use_lds, out);
     return hipGetLastError();
 }
 
@@ -951,12 +1023,12 @@ hipError_t launch_prep_batch(const PrepSeriesDev* d_ps, int nseries, int64_t max
             hipLaunchKernelGGL(prep_scrunch_kernel, dim3(grid_for(max_lores, 64, 4096), nseries),
                                dim3(64), 0, stream, d_ps);
         }
-        // the window of a workgroup in LDS when it fits in 48 KiB
-        const size_t span = (size_t)256 + max_width - 1;
-        const int use_lds = span * sizeof(float) <= 48 * 1024;
+        // the sorted path when its LDS fits in 48 KiB
+        const size_t lds = rmed_lds_floats(256, max_width) * sizeof(float);
+        const int use_lds = lds <= 48 * 1024;
         hipLaunchKernelGGL(prep_rmed_kernel,
                            dim3(grid_for(std::max(max_lores, (int64_t)1), 256, 4096), nseries),
-                           dim3(256), use_lds ? span * sizeof(float) : 0, stream, d_ps, use_lds);
+                           dim3(256), use_lds ? lds : 0, stream, d_ps, use_lds);
         hipLaunchKernelGGL(prep_subtract_kernel, dim3(grid_for((max_n + 3) / 4, 256, 2048), nseries), dim3(256),
                            0, stream, d_ps);
     }

@@ -639,10 +639,10 @@ __device__ __forceinline__ void window_max_default(const float* cps, int i0, int
 //     into C chunks; one thread per (chunk, row), chunk-major so that the
 //     lanes of a wavefront take the same chunk of consecutive rows, with C
 //     chosen so that about every thread of the workgroup has an item.
-//     Chunk totals in float-float go to LDS, then every item rescans its
-//     chunk from the float-float sum of the chunks before it (which it adds
-//     up itself, in chunk order), writing each partial sum rounded once to
-//     float32 -- what the reference's float64 accumulator gives.
+//     Chunk totals in float64 go to LDS, then every item rescans its chunk
+//     from the float64 sum of the chunks before it (which it adds up itself,
+//     in chunk order), writing each partial sum rounded to float32, as the
+//     reference's float64 accumulator does.
 //  2. window maxima: the same (chunk, row) threads, all widths per thread
 //     (each prefix value read once per width), combined across chunks with
 //     LDS integer max; then one thread per (row, width) writes the S/N.
@@ -664,10 +664,12 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
     // columns (the last chunk also takes the remainder); the lanes of one
     // wavefront work on consecutive rows of one or two chunks.
     int C = max(1, NT / nrows);
-    C = min(C, max(1, bins / 8));                      // chunks of >= 8 columns
-    C = min(C, max(1, scratch_floats / (2 * nrows)));  // chunk totals fit
+    C = min(C, max(1, bins / 8));                            // chunks of >= 8 columns
+    C = min(C, max(1, (scratch_floats - 1) / (2 * nrows)));  // chunk totals fit
     const int nitems = C * nrows;
-    float* tot = scratch;  // [C][nrows] float-float pairs
+    // [C][nrows] float64 chunk totals, 8-byte aligned
+    double* tot = reinterpret_cast<double*>((reinterpret_cast<uintptr_t>(scratch) + 7) &
+                                            ~static_cast<uintptr_t>(7));
     // widths and (h + b, b) pairs of the transform: loaded now, stored to
     // LDS after the first prefix pass, so that their latency is hidden
     const int nwl = min(nw, kSnrMaxWidths);
@@ -678,53 +680,40 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         if (lane < 2 * nwl) hb_ld = as_global(tk.hb)[lane];
     }
     const int g4 = bins >> 2;
-    // (1) chunk totals in float-float: the running float sum s and the sum
-    // l of its exact rounding errors
+    // (1) chunk totals, accumulated in float64 as the reference does
     for (int it = tid; it < nitems; it += NT) {
         const int c = it / nrows;
         const int p = it - c * nrows;
         const float* row = data + (size_t)p * pitch;
         int j = 4 * ((c * g4) / C);
         const int je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
-        float sh = 0.0f, sl = 0.0f;
+        double acc = 0.0;
         for (; j + 4 <= je; j += 4) {
             float x[4];
 #pragma unroll
             for (int u = 0; u < 4; ++u) x[u] = row[j + u];
             asm volatile("" ::: "memory");
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                float t, e;
-                two_sum(sh, x[u], t, e);
-                sh = t;
-                sl += e;
-            }
+            for (int u = 0; u < 4; ++u) acc += (double)x[u];
         }
-        for (; j < je; ++j) {
-            float t, e;
-            two_sum(sh, row[j], t, e);
-            sh = t;
-            sl += e;
-        }
-        tot[2 * it] = sh;
-        tot[2 * it + 1] = sl;
+        for (; j < je; ++j) acc += (double)row[j];
+        tot[it] = acc;
     }
     if (wave == NW - 1) {
         if (lane < nwl) s_w[lane] = w_ld;
         if (lane < 2 * nwl) s_hb[lane] = hb_ld;
     }
     __syncthreads();
-    // (2) every chunk rescanned from the sum of the chunks before it (added
-    // in chunk order by the item itself), each partial sum rounded once to
-    // float32 -- what the reference's float64 accumulator gives.  The chunk
-    // totals stay in place: items only read them, so no barrier is needed
-    // before the rows are overwritten.
+    // (2) every chunk rescanned in float64 from the sum of the chunks before
+    // it (added in chunk order by the item itself), each partial sum rounded
+    // to float32 (kernels.hpp:75-83).  The chunk totals stay in place: items
+    // only read them, so no barrier is needed before the rows are
+    // overwritten.
     for (int it = tid; it < nitems; it += NT) {
         const int c = it / nrows;
         const int p = it - c * nrows;
-        float sh = 0.0f, sl = 0.0f;
-        for (int q = 0; q < c; ++q)
-            ff_add(sh, sl, tot[2 * (q * nrows + p)], tot[2 * (q * nrows + p) + 1], sh, sl);
+        double acc = 0.0;
+        for (int q = 0; q < c; ++q) acc += tot[q * nrows + p];
         float* row = data + (size_t)p * pitch;
         int j = 4 * ((c * g4) / C);
         const int je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
@@ -735,21 +724,15 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             asm volatile("" ::: "memory");
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
-                float t, e;
-                two_sum(sh, x[u], t, e);
-                sh = t;
-                sl += e;
-                x[u] = sh + sl;  // rounded partial sum
+                acc += (double)x[u];
+                x[u] = (float)acc;
             }
 #pragma unroll
             for (int u = 0; u < 4; ++u) row[j + u] = x[u];
         }
         for (; j < je; ++j) {
-            float t, e;
-            two_sum(sh, row[j], t, e);
-            sh = t;
-            sl += e;
-            row[j] = sh + sl;
+            acc += (double)row[j];
+            row[j] = (float)acc;
         }
     }
     __syncthreads();

@@ -14,6 +14,9 @@ constexpr int kDsMaxSteps = 256;
 // kernel.
 constexpr int kDsChunk = 4096;
 constexpr double kDsPackedMaxF = 4000.0;
+// Staged samples per thread loaded together (a chunk plus a halo of up to
+// 1024 samples; a longer halo takes a loop for the rest).
+constexpr int kDsStageLoads = (kDsChunk + 1024 + kDsThreads - 1) / kDsThreads;
 
 // One downsampling step: f = mf * 2^ef exactly (the float64 factor split
 // into integer significand and exponent), inv_f ~ 1/f for index estimates.

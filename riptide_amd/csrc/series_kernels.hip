@@ -94,6 +94,13 @@ __device__ __forceinline__ int64_t ds_first_at(int64_t c, const DsStepDev& st)
 // wmin*in[ia], the interior samples left to right, then the last one fused.
 // The first NI interior samples are loaded together (a factor f has at most
 // floor(f) + 1 of them; more than NI take the loop at the end).
+// Staged sample i sits at LDS word dsp(i): one pad word per 32 samples, so
+// that the lanes of a wavefront reading samples about f apart (consecutive
+// outputs of a factor f) fall on different banks.
+__host__ __device__ inline int dsp(int i) { return i + (i >> 5); }
+// LDS words of an array of n padded samples.
+__host__ __device__ inline int ds_words(int n) { return n + (n >> 5) + 1; }
+
 template <int NI>
 __device__ __forceinline__ float ds_direct(const float* in, int ia, int ib, float wmin, float wmax)
 {
@@ -101,13 +108,13 @@ __device__ __forceinline__ float ds_direct(const float* in, int ia, int ib, floa
     const int last = n > 1 ? n - 1 : 0;
     float t[NI];
 #pragma unroll
-    for (int j = 0; j < NI; ++j) t[j] = in[ia + 1 + min(j, last)];
-    float acc = wmin * in[ia];
+    for (int j = 0; j < NI; ++j) t[j] = in[dsp(ia + 1 + min(j, last))];
+    float acc = wmin * in[dsp(ia)];
 #pragma unroll
     for (int j = 0; j < NI; ++j)
         if (j < n) acc += t[j];
-    for (int i = ia + 1 + NI; i < ib; ++i) acc += in[i];
-    return __fmaf_rn(wmax, in[ib], acc);
+    for (int i = ia + 1 + NI; i < ib; ++i) acc += in[dsp(i)];
+    return __fmaf_rn(wmax, in[dsp(ib)], acc);
 }
 
 // All downsampling steps of a search in one pass over the raw series: every
@@ -136,23 +143,44 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
     __shared__ float* s_out[kDsMaxSteps];
     const int cap = chunk + halo;
     float* s_in = s_mem;             // staged samples
-    float* s_ch = s_in + cap + 1;    // exclusive prefix, high parts (cap + 1)
-    float* s_cl = s_ch + cap + 1;    // low parts
+    float* s_ch = s_in + ds_words(cap + 1);  // exclusive prefix, high parts (cap + 1)
+    float* s_cl = s_ch + ds_words(cap + 1);  // low parts
 
-    // find the series of this chunk
-    int lo = 0, hi = nseries - 1;
+    // find the series of this chunk: the proportional guess (exact when the
+    // series have equal chunk counts), else a binary search
     const int b = blockIdx.x;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (series[mid].chunk_offset <= b) lo = mid;
-        else hi = mid - 1;
+    int lo = (int)min((int64_t)b * nseries / (int64_t)gridDim.x, (int64_t)nseries - 1);
+    if (!(series[lo].chunk_offset <= b &&
+          (lo == nseries - 1 || b < series[lo + 1].chunk_offset))) {
+        int hi = nseries - 1;
+        lo = 0;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (series[mid].chunk_offset <= b) lo = mid;
+            else hi = mid - 1;
+        }
     }
     const DsSeriesDev sr = series[lo];
     const int64_t c0 = (int64_t)(b - sr.chunk_offset) * chunk;
     const int64_t c1 = min(c0 + chunk, sr.N);
     const int L = (int)(min(c0 + chunk + halo, sr.N) - c0);
     const int tid = threadIdx.x;
-    for (int j = tid; j < L; j += blockDim.x) s_in[j] = sr.x[c0 + j];
+    {
+        // every load of the chunk issued before the first LDS store
+        constexpr int R = kDsStageLoads;
+        float v[R];
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int j = tid + u * kDsThreads;
+            if (j < L) v[u] = sr.x[c0 + j];
+        }
+#pragma unroll
+        for (int u = 0; u < R; ++u) {
+            const int j = tid + u * kDsThreads;
+            if (j < L) s_in[dsp(j)] = v[u];
+        }
+        for (int j = tid + R * kDsThreads; j < L; j += kDsThreads) s_in[dsp(j)] = sr.x[c0 + j];
+    }
 
     const DsStepDev* st = steps + sr.step0;
     for (int s = tid; s < sr.nsteps; s += blockDim.x) {
@@ -171,7 +199,7 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
     float sh = 0.0f, sl = 0.0f;
     for (int j = j0; j < j1; ++j) {
         float t, e;
-        two_sum(sh, s_in[j], t, e);
+        two_sum(sh, s_in[dsp(j)], t, e);
         sh = t;
         sl += e;
     }
@@ -222,16 +250,16 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
     sh = s_th[tid];
     sl = s_tl[tid];
     for (int j = j0; j < j1; ++j) {
-        s_ch[j] = sh + sl;
-        s_cl[j] = sl - ((sh + sl) - sh);
+        s_ch[dsp(j)] = sh + sl;
+        s_cl[dsp(j)] = sl - ((sh + sl) - sh);
         float t, e;
-        two_sum(sh, s_in[j], t, e);
+        two_sum(sh, s_in[dsp(j)], t, e);
         sh = t;
         sl += e;
     }
     if (j1 == L && j0 < j1) {
-        s_ch[L] = sh + sl;
-        s_cl[L] = sl - ((sh + sl) - sh);
+        s_ch[dsp(L)] = sh + sl;
+        s_cl[dsp(L)] = sl - ((sh + sl) - sh);
     }
     __syncthreads();
 
@@ -272,8 +300,8 @@ __global__ __launch_bounds__(kDsThreads) void downsample_packed_kernel(
         } else {
             float inner = 0.0f;
             if (ib > ia + 1)
-                inner = (s_ch[ib] - s_ch[ia + 1]) + (s_cl[ib] - s_cl[ia + 1]);
-            v = __fmaf_rn(ix.wmax, s_in[ib], ix.wmin * s_in[ia] + inner);
+                inner = (s_ch[dsp(ib)] - s_ch[dsp(ia + 1)]) + (s_cl[dsp(ib)] - s_cl[dsp(ia + 1)]);
+            v = __fmaf_rn(ix.wmax, s_in[dsp(ib)], ix.wmin * s_in[dsp(ia)] + inner);
         }
         orow[k] = v;
     }
@@ -953,7 +981,7 @@ hipError_t launch_downsample_packed(const DsSeriesDev* d_series, int nseries,
                                     int halo, hipStream_t stream)
 {
     // staged samples and the two prefix arrays
-    const size_t lds = (size_t)3 * (chunk + halo + 1) * sizeof(float);
+    const size_t lds = (size_t)3 * ds_words(chunk + halo + 1) * sizeof(float);
     static size_t configured = 0;
     if (lds > 48 * 1024 && lds > configured) {
         const hipError_t e = hipFuncSetAttribute((const void*)downsample_packed_kernel,

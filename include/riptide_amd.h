@@ -222,6 +222,15 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                     int deredden, double rmed_width, int64_t rmed_minpts, int normalise,
                     int64_t group_size, void** handle);
 
+/* Tiling of the batch's transforms by (rows, bins) class, for batches created
+ * afterwards: n entries of 7 int32 each, {rows_lo, rows_hi, bins_lo, bins_hi,
+ * shape, cap, lmax}; a transform of r rows and b bins takes the first entry
+ * with rows_lo <= r < rows_hi and bins_lo <= b < bins_hi: FFA workgroup
+ * shape, rows per tile (0: as many as fit) and merge levels per cone pass
+ * (0: the planner's limit).  Transforms no entry holds take the current
+ * shape (rt_set_ffa_shape) with the largest tile.  n = 0 clears the table. */
+int rt_batch_set_tiling(const int32_t* entries, int n);
+
 /* Number of trial periods of series i, and its period / fold-bins grid. */
 int rt_batch_length(void* handle, int64_t i, int64_t* length);
 int rt_batch_grid(void* handle, int64_t i, double* h_periods, uint32_t* h_foldbins);

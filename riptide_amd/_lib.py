@@ -61,6 +61,7 @@ _SIGNATURES = {
     "rt_batch_run_timed": [_c_ptr, _c_ptr, _c_ptr],
     "rt_batch_check": [_c_ptr, _c_ptr],
     "rt_batch_stats": [_c_ptr, _c_ptr],
+    "rt_batch_set_tiling": [_c_ptr, ctypes.c_int],
     "rt_batch_destroy": [_c_ptr],
     "rt_grad_batch_create": [_c_i64, _c_i64, _c_dbl, _c_ptr, _c_i64, _c_dbl, _c_dbl, _c_i64,
                              _c_i64, _c_ptr],

@@ -21,7 +21,8 @@ class BatchSearch:
 
     def __init__(self, sizes, tsamps, bins_min, bins_max, period_min=1.0, period_max=30.0,
                  widths=None, ducy_max=0.20, wtsp=1.5, deredden=True, rmed_width=4.0,
-                 rmed_minpts=101, already_normalised=False, group_size=64, shape=None):
+                 rmed_minpts=101, already_normalised=False, group_size=64, shape=None,
+                 tiling=None):
         n = len(sizes)
         self.n = n
         self.sizes = np.ascontiguousarray(sizes, dtype=np.int64)
@@ -37,14 +38,19 @@ class BatchSearch:
         self.period_min = float(period_min)
         self.period_max = float(period_max)
         h = ctypes.c_void_p()
-        # FFA workgroup shape: explicit, else the autotuner's choice for the
-        # widest bins class of the batch (its transforms cost the most;
-        # riptide_amd.tune), else the library default
-        if shape is None:
-            shape = tune.shape_for("batch", int(self.bins_min.max()))
+        # FFA tiling: an explicit shape for every transform, else explicit
+        # tiling entries [rows_lo, rows_hi, bins_lo, bins_hi, shape, cap,
+        # lmax], else the autotuner's per-(rows, bins)-class table
+        # (riptide_amd.tune), else the library default
         self.shape = shape
         if shape is not None:
+            tiling = []
+        elif tiling is None:
+            tiling = tune.batch_tiling()
+        self.tiling = [list(map(int, e)) for e in tiling]
+        if shape is not None:
             _lib.call("rt_set_ffa_shape", int(shape))
+        tune.set_batch_tiling(self.tiling)
         try:
             _lib.call("rt_batch_create", n, _lib.host_ptr(self.sizes), _lib.host_ptr(self.tsamps),
                       _lib.host_ptr(flat), _lib.host_ptr(counts), self.period_min,
@@ -52,6 +58,7 @@ class BatchSearch:
                       int(deredden), float(rmed_width), int(rmed_minpts),
                       int(not already_normalised), int(group_size), ctypes.byref(h))
         finally:
+            tune.set_batch_tiling([])
             if shape is not None:
                 tune.restore()
         self.handle = h

@@ -54,8 +54,44 @@ __global__ void fill_int_kernel(int* p, int n, int v)
 struct Run {
     int tr0 = 0, ntr = 0;          // transforms in the device table
     int variant = 0;
+    int shape = 0;                 // workgroup shape of the launch
     int queue = 0;                 // queue counter index
 };
+
+// Tiling of the transforms of one (rows, bins) class, chosen by measurement:
+// workgroup shape, rows per tile (0: as many as fit) and merge levels per
+// cone pass (0: the planner's limit).
+struct TileClass {
+    int32_t rows_lo, rows_hi;      // rows in [rows_lo, rows_hi)
+    int32_t bins_lo, bins_hi;      // bins in [bins_lo, bins_hi)
+    int32_t shape, cap, lmax;
+};
+std::vector<TileClass> g_tile_classes;
+
+// The tiling of a transform: the first class that holds it, else the
+// current shape with the largest tile.
+void tile_for(int64_t rows, int64_t bins, int default_shape, int* shape, int* cap, int* lmax)
+{
+    *shape = default_shape;
+    *cap = 0;
+    *lmax = kMaxConeLevels;
+    for (const TileClass& c : g_tile_classes) {
+        if (rows >= c.rows_lo && rows < c.rows_hi && bins >= c.bins_lo && bins < c.bins_hi) {
+            *shape = c.shape;
+            *cap = c.cap;
+            *lmax = c.lmax > 0 ? c.lmax : kMaxConeLevels;
+            break;
+        }
+    }
+    const int most = ffa_cap_rows((int)bins, *shape);
+    *cap = (*cap > 0 && *cap < most) ? *cap : most;
+}
+
+// Workgroups of a shape resident at once: as many per CU as its LDS allows.
+int shape_slots(int cus, int shape)
+{
+    return std::max(1, cus) * std::max(1, (160 * 1024) / (kFfaShape[shape].lds_bytes + 1024));
+}
 
 struct Group {
     int s0 = 0, s1 = 0;            // series [s0, s1)
@@ -114,6 +150,7 @@ struct Batch {
     int* d_err = nullptr;
     int64_t ds_floats = 0, prep_floats = 0, slot_floats = 0, work_floats = 0;
     int slots = 0;
+    int cus = 1;
     int shape = 0;
     // bound buffers
     std::vector<const float*> inputs;
@@ -304,19 +341,21 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                 f.rows_eval = t.rows_eval;
                 f.nw = (int32_t)b->wlists[wg].size();
                 f.stdnoise = t.stdnoise;
-                f.cap = ffa_cap_rows(t.bins, b->shape);
-                f.shape = b->shape;
+                int tshape = 0, tcap = 0, tlmax = 0;
+                tile_for(t.rows, t.bins, b->shape, &tshape, &tcap, &tlmax);
+                f.cap = tcap;
+                f.shape = tshape;
                 f.series = s;
                 if (t.rows <= f.cap) {
                     f.npass = 1;
                     f.levels0 = ffa_height(t.rows);
                 } else {
-                    const int np = count_transform_passes(t.rows, t.bins, f.cap);
+                    const int np = count_transform_passes(t.rows, t.bins, f.cap, tlmax);
                     if (np < 0 || np > kMaxPasses) {
                         free_batch(b);
                         return bfail(RT_EINVAL, "bins too large for the FFA tiling");
                     }
-                    std::vector<int> depths = transform_pass_depths(t.rows, f.cap);
+                    std::vector<int> depths = transform_pass_depths(t.rows, f.cap, tlmax);
                     f.npass = (int32_t)depths.size();
                     for (size_t q = 0; q < depths.size(); ++q) f.depth[q] = depths[q];
                     max_fold = std::max<int64_t>(max_fold, (int64_t)t.rows * t.bins);
@@ -327,20 +366,24 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                 local_pidx.push_back((int)it);
             }
         }
-        // by kernel variant, then largest first
+        // by kernel variant and workgroup shape, then largest first
         std::stable_sort(order.begin(), order.end(),
                          [&local](const std::pair<int64_t, int>& a, const std::pair<int64_t, int>& c) {
                              const int va = ffa_variant(local[a.second].bins);
                              const int vc = ffa_variant(local[c.second].bins);
                              if (va != vc) return va < vc;
+                             if (local[a.second].shape != local[c.second].shape)
+                                 return local[a.second].shape < local[c.second].shape;
                              return a.first > c.first;
                          });
         for (const auto& o : order) {
             const int v = ffa_variant(local[o.second].bins);
-            if (g.runs.empty() || g.runs.back().variant != v) {
+            const int sh = local[o.second].shape;
+            if (g.runs.empty() || g.runs.back().variant != v || g.runs.back().shape != sh) {
                 Run r;
                 r.tr0 = (int)b->host_trs.size();
                 r.variant = v;
+                r.shape = sh;
                 r.queue = b->nqueues++;
                 g.runs.push_back(r);
             }
@@ -384,8 +427,11 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
 
     int cus = 0;
     BT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
-    // workgroups resident at once: as many per CU as the LDS of the shape allows
-    b->slots = std::max(1, cus) * std::max(1, (160 * 1024) / (kFfaShape[b->shape].lds_bytes + 1024));
+    // scratch slots: the most workgroups any run's shape keeps resident
+    b->cus = std::max(1, cus);
+    b->slots = 1;
+    for (const Group& g : b->groups)
+        for (const Run& r : g.runs) b->slots = std::max(b->slots, shape_slots(cus, r.shape));
     b->ds_floats = max_ds;
     b->prep_floats = (deredden || normalise) ? max_prep : 1;
     b->slot_floats = 2 * max_fold;
@@ -519,7 +565,8 @@ static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 2], s));
         for (const Run& r : g.runs)
             BT_HIP(launch_ffa_batch(b->d_trs + r.tr0, r.ntr, b->d_queue + r.queue, b->d_scratch,
-                                    b->slot_floats, b->slots, b->d_err, s, b->shape, r.variant));
+                                    b->slot_floats, shape_slots(b->cus, r.shape), b->d_err, s,
+                                    r.shape, r.variant));
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 3], s));
     }
     return RT_OK;
@@ -591,6 +638,23 @@ int rt_batch_check(void* handle, void* stream)
     BT_HIP(hipMemcpy(&herr, b->d_err, sizeof(int), hipMemcpyDeviceToHost));
     if (herr & 1) return bfail(RT_EDEVICE, "FFA tile exceeded its LDS budget");
     if (herr) return bfail(RT_EDEVICE, "FFA transform launched with the wrong kernel variant");
+    return RT_OK;
+}
+
+int rt_batch_set_tiling(const int32_t* entries, int n)
+{
+    if (n < 0 || (n > 0 && entries == nullptr)) return bfail(RT_EINVAL, "bad tiling table");
+    std::vector<TileClass> t;
+    for (int i = 0; i < n; ++i) {
+        const int32_t* e = entries + 7 * i;
+        TileClass c = {e[0], e[1], e[2], e[3], e[4], e[5], e[6]};
+        if (c.rows_lo < 0 || c.rows_hi <= c.rows_lo || c.bins_lo < 1 || c.bins_hi <= c.bins_lo ||
+            c.shape < 0 || c.shape >= kFfaShapes || c.cap < 0 || c.lmax < 0 ||
+            c.lmax > kMaxConeLevels)
+            return bfail(RT_EINVAL, "bad tiling table entry");
+        t.push_back(c);
+    }
+    g_tile_classes = t;
     return RT_OK;
 }
 

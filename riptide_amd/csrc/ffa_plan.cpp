@@ -57,8 +57,9 @@ static int leaf_depth(int64_t m, int cap)
 // internal segment grows by at most 2 rows per level.
 static int64_t tile_rows(int cap, int L) { return (int64_t)cap - 2 * (((int64_t)1 << L) - 1); }
 
-// Chooses the depths d_0 = d1 > d_1 > ... > d_P = 0 of the cone passes.
-static std::vector<int> cone_depths(int d1, int cap)
+// Chooses the depths d_0 = d1 > d_1 > ... > d_P = 0 of the cone passes, at
+// most lmax levels per pass.
+static std::vector<int> cone_depths(int d1, int cap, int lmax)
 {
     std::vector<int> best;
     double best_cost = 1e300;
@@ -70,7 +71,7 @@ static std::vector<int> cone_depths(int d1, int cap)
         int rem = d1;
         for (int p = 0; p < P; ++p) {
             const int L = (rem + (P - p) - 1) / (P - p);
-            if (L > kMaxConeLevels) { ok = false; break; }
+            if (L > lmax || L > kMaxConeLevels) { ok = false; break; }
             const int64_t K = tile_rows(cap, L);
             if (K < 4) { ok = false; break; }
             cost += (double)(K + 2 * ((1 << L) - 1)) / (double)K;  // reads per output row
@@ -87,10 +88,10 @@ static std::vector<int> cone_depths(int d1, int cap)
     return best;
 }
 
-std::vector<int> transform_pass_depths(int64_t m, int cap)
+std::vector<int> transform_pass_depths(int64_t m, int cap, int lmax)
 {
     if (m <= cap) return {0};
-    return cone_depths(leaf_depth(m, cap), cap);
+    return cone_depths(leaf_depth(m, cap), cap, lmax);
 }
 
 int64_t ffa_adds(int64_t m)
@@ -100,12 +101,12 @@ int64_t ffa_adds(int64_t m)
     return ffa_adds(mh) + ffa_adds(m - mh) + m;
 }
 
-int count_transform_passes(int64_t m, int bins, int cap)
+int count_transform_passes(int64_t m, int bins, int cap, int lmax)
 {
     (void)bins;
     if (m <= cap) return 1;
     const int d1 = leaf_depth(m, cap);
-    const std::vector<int> depths = cone_depths(d1, cap);
+    const std::vector<int> depths = cone_depths(d1, cap, lmax);
     if (depths.empty()) return -1;
     return (int)depths.size();  // leaf pass + (size-1) cone passes
 }
@@ -145,7 +146,7 @@ int plan_transform_tasks(const TransformSpec& s, std::vector<std::vector<FfaTask
         return 1;
     }
     const int d1 = leaf_depth(s.m, s.cap);
-    const std::vector<int> depths = cone_depths(d1, s.cap);
+    const std::vector<int> depths = cone_depths(d1, s.cap, kMaxConeLevels);
     if (depths.empty()) return -1;
     const int npass = (int)depths.size();
     ensure((size_t)npass);

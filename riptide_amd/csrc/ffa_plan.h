@@ -52,13 +52,14 @@ int plan_transform_tasks(const TransformSpec& s, std::vector<std::vector<FfaTask
 
 // Depths of the pass split of an m-row transform: {d1, ..., 0} (pass 0 builds
 // whole nodes of depth d1, pass p the nodes of depth depths[p]); {0} when the
-// transform fits one workgroup; empty when it cannot be tiled.
-std::vector<int> transform_pass_depths(int64_t m, int cap);
+// transform fits one workgroup; empty when it cannot be tiled.  At most lmax
+// merge levels per cone pass.
+std::vector<int> transform_pass_depths(int64_t m, int cap, int lmax = kMaxConeLevels);
 
 // Row additions of an m-row FFA (multiply by the bins for float adds).
 int64_t ffa_adds(int64_t m);
 
 // Number of passes plan_transform_tasks would use (no tasks built).
-int count_transform_passes(int64_t m, int bins, int cap);
+int count_transform_passes(int64_t m, int bins, int cap, int lmax = kMaxConeLevels);
 
 }  // namespace rt

@@ -1,20 +1,25 @@
-"""Autotuner of the FFA kernels' workgroup shape.
+"""Autotuner of the FFA kernels' tiling.
 
 The FFA task kernels come in a few workgroup shapes (threads per workgroup,
 LDS per workgroup, merge-level values each lane keeps in registers; see
 ``kFfaShape`` in csrc/tasks.h).  The shape fixes how many rows a workgroup
 tiles (so how many passes a transform needs) and how many workgroups share a
-compute unit.  Which one is fastest depends on the bins range, the series
-length and the schedule (one series or a batch), so it is measured:
+compute unit; a transform taller than a tile is computed in passes of cone
+tiles, each at most ``lmax`` merge levels deep.  Which choice is fastest
+depends on the bins range, the transform's rows and the schedule (one series
+or a batch), so it is measured:
 
     python -m riptide_amd.tune [--nsamp N] [--tsamp T] [--trials K] [--out FILE]
 
-times, for every shape and bins class, (a) the single-series periodogram
-and (b) a batch of K series, on synthetic white noise of the given shape,
+times, on synthetic white noise of the given shape,
+ (a) the single-series periodogram under every shape, per bins class;
+ (b) a batch of K series under every (shape, lmax), per (rows class, bins
+     class): the period range is cut so that the batch's transforms have
+     rows in the class (rows ~ nsamp * tsamp / period);
 and writes the winners to a JSON file (default: ``tuning.json`` next to this
-module, or ``$RIPTIDE_AMD_TUNING``).  ``shape_for(kind, bins_min)`` reads it;
-``BatchSearch`` and the periodogram entry points apply it when no shape was
-set explicitly with ``set_shape``.
+module, or ``$RIPTIDE_AMD_TUNING``).  ``shape_for("periodogram", bins_min)``
+and ``batch_tiling()`` read it; the periodogram entry points and
+``BatchSearch`` apply it unless a shape was forced with ``set_shape``.
 """
 import argparse
 import json
@@ -27,6 +32,10 @@ from . import _lib
 
 NUM_SHAPES = 3
 DEFAULT_CLASSES = ((240, 260), (480, 520))
+# rows classes of the batch tiling: [lo, hi)
+ROWS_CLASSES = ((1, 128), (128, 1024), (1024, 1 << 30))
+# merge levels per cone pass tried by the batch tuner (the planner allows 7)
+LMAX_CHOICES = (2, 3, 4, 5)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 _cache = None
@@ -66,8 +75,8 @@ def load(path=None):
 
 
 def shape_for(kind, bins_min):
-    """Tuned shape for schedule `kind` ("batch" or "periodogram") and the bins
-    class holding bins_min (the class with the nearest lower bound), or None."""
+    """Tuned shape for schedule `kind` ("periodogram") and the bins class
+    holding bins_min (the class with the nearest lower bound), or None."""
     if _forced is not None:
         return _forced
     best = load().get(kind, {})
@@ -83,6 +92,26 @@ def apply(kind, bins_min):
     if s is not None and _forced is None:
         _lib.call("rt_set_ffa_shape", s)
     return s
+
+
+def batch_tiling():
+    """Tiling entries [rows_lo, rows_hi, bins_lo, bins_hi, shape, cap, lmax]
+    of the tuning table for BatchSearch (empty without a table, or when a
+    shape is forced).  A bins class [lo, hi] covers bins lo .. hi inclusive."""
+    if _forced is not None:
+        return []
+    out = []
+    for e in load().get("batch_tiles", []):
+        out.append([int(e["rows"][0]), int(e["rows"][1]), int(e["bins"][0]),
+                    int(e["bins"][1]) + 1, int(e["shape"]), int(e.get("cap", 0)),
+                    int(e.get("lmax", 0))])
+    return out
+
+
+def set_batch_tiling(entries):
+    """Hands tiling entries to the library for the next rt_batch_create."""
+    arr = np.ascontiguousarray(np.asarray(entries, dtype=np.int32).reshape(-1, 7))
+    _lib.call("rt_batch_set_tiling", _lib.host_ptr(arr), int(arr.shape[0]))
 
 
 def _time_periodogram(shape, nsamp, tsamp, bins, pmin, pmax, reps=5):
@@ -105,13 +134,15 @@ def _time_periodogram(shape, nsamp, tsamp, bins, pmin, pmax, reps=5):
     return float(np.median(ts))
 
 
-def _time_batch(shape, nsamp, tsamp, bins, pmin, pmax, trials):
+def _time_batch(shape, lmax, nsamp, tsamp, bins, pmin, pmax, trials):
+    """Least of two FFA phase times of a batch of `trials` series, every
+    transform tiled with (shape, lmax)."""
     import torch
     from .batch import BatchSearch
-    set_shape(shape)
     xs = [torch.randn(nsamp, device=_lib.device()) for _ in range(trials)]
     b = BatchSearch([nsamp] * trials, tsamp, bins[0], bins[1], pmin, pmax, deredden=False,
-                    already_normalised=True, group_size=trials, shape=shape)
+                    already_normalised=True, group_size=trials,
+                    tiling=[[0, 1 << 30, 1, 1 << 30, shape, 0, lmax]])
     outs = b.allocate_outputs()
     b.bind(xs, outs)
     b.run_timed()
@@ -120,13 +151,22 @@ def _time_batch(shape, nsamp, tsamp, bins, pmin, pmax, trials):
     return ms
 
 
+def rows_period_range(rows, nsamp, tsamp, pmin, pmax):
+    """Period range whose transforms have about `rows` = [lo, hi) rows
+    (rows ~ nsamp * tsamp / period), within [pmin, pmax]; None if empty."""
+    span = nsamp * tsamp
+    p0 = max(pmin, span / rows[1])
+    p1 = min(pmax, span / rows[0])
+    return (p0, p1) if p1 > p0 * 1.001 else None
+
+
 def tune(nsamp=2 ** 22, tsamp=64e-6, period_min=0.1, period_max=10.0, trials=16,
          classes=DEFAULT_CLASSES, out=None, log=print):
     """Measures every shape on every bins class and writes the table."""
     table = {"device": None, "workload": {"nsamp": nsamp, "tsamp": tsamp,
                                           "period_min": period_min, "period_max": period_max,
                                           "trials": trials},
-             "periodogram": {}, "batch": {}, "timings_ms": {}}
+             "periodogram": {}, "timings_ms": {}}
     try:
         import torch
         table["device"] = torch.cuda.get_device_name(0)
@@ -142,20 +182,30 @@ def tune(nsamp=2 ** 22, tsamp=64e-6, period_min=0.1, period_max=10.0, trials=16,
                          for s in range(NUM_SHAPES)]
     finally:
         set_shape(None)
+    table["batch_tiles"] = []
     for bins in classes:
         key = f"{bins[0]}-{bins[1]}"
         pg = pgs[bins]
-        try:
-            bt = [_time_batch(s, nsamp, tsamp, bins, period_min, period_max, trials)
-                  for s in range(NUM_SHAPES)]
-        finally:
-            set_shape(None)
         table["periodogram"][key] = int(np.argmin(pg))
-        table["batch"][key] = int(np.argmin(bt))
-        table["timings_ms"][key] = {"periodogram": [round(v, 3) for v in pg],
-                                    "batch_ffa": [round(v, 3) for v in bt]}
-        log(f"bins {key}: periodogram ms by shape {table['timings_ms'][key]['periodogram']}, "
-            f"batch of {trials} ffa ms by shape {table['timings_ms'][key]['batch_ffa']}")
+        table["timings_ms"][key] = {"periodogram": [round(v, 3) for v in pg]}
+        log(f"bins {key}: periodogram ms by shape {table['timings_ms'][key]['periodogram']}")
+        for rows in ROWS_CLASSES:
+            pr = rows_period_range(rows, nsamp, tsamp, period_min, period_max)
+            if pr is None:
+                continue
+            ms = {}
+            for shape in range(NUM_SHAPES):
+                for lmax in LMAX_CHOICES:
+                    ms[(shape, lmax)] = _time_batch(shape, lmax, nsamp, tsamp, bins, pr[0],
+                                                    pr[1], trials)
+            best = min(ms, key=ms.get)
+            table["batch_tiles"].append({
+                "rows": [rows[0], rows[1]], "bins": [bins[0], bins[1]], "shape": best[0],
+                "cap": 0, "lmax": best[1], "periods": [round(pr[0], 6), round(pr[1], 6)],
+                "ffa_ms": {f"shape{k[0]}_lmax{k[1]}": round(v, 3) for k, v in ms.items()}})
+            log(f"bins {key}, rows {rows[0]}-{rows[1]} (P {pr[0]:.3g}-{pr[1]:.3g} s): best "
+                f"shape {best[0]}, lmax {best[1]}: {ms[best]:.2f} ms; shape 0, lmax 5: "
+                f"{ms[(0, 5)]:.2f} ms")
     set_shape(None)
     path = out or tuning_path()
     with open(path, "w") as f:

@@ -60,3 +60,43 @@ def test_ffa_search_batch_api(oracle):
                                rmed_width=1.0)
         assert np.array_equal(pg.periods, pg1.periods)
         np.testing.assert_allclose(pg.snrs, pg1.snrs, rtol=1e-5, atol=1e-4)
+
+
+def test_batch_tiling_classes_match_oracle(oracle):
+    """Transforms of different (rows, bins) classes tiled with different
+    workgroup shapes and cone depths in one batch (several runs per kernel
+    variant): every series still matches the oracle."""
+    import torch
+    from riptide_amd import _lib
+    from riptide_amd.batch import BatchSearch
+
+    rng = np.random.default_rng(77)
+    tsamp = 128e-6
+    sizes = [2 ** 18, 2 ** 18 + 5000, 2 ** 17 + 123, 2 ** 18 - 777]
+    bmin = np.array([240, 480, 240, 480])
+    bmax = np.where(bmin == 240, 260, 520)
+    data = [_series(rng, n, tsamp) for n in sizes]
+    tiling = [[1, 60, 240, 261, 1, 0, 2],
+              [60, 300, 240, 261, 2, 40, 3],
+              [300, 1 << 30, 240, 261, 0, 0, 2],
+              [1, 200, 480, 521, 2, 0, 4],
+              [200, 1 << 30, 480, 521, 1, 0, 2]]
+    b = BatchSearch(sizes, tsamp, bmin, bmax, 0.08, 2.0, rmed_width=1.0, group_size=4,
+                    tiling=tiling)
+    xs = [_lib.to_device(x) for x in data]
+    outs = b.allocate_outputs()
+    b.bind(xs, outs)
+    for use_graph in (False, True):
+        for o in outs:
+            o.fill_(np.nan)
+        b.run(use_graph=use_graph)
+        b.check()
+        torch.cuda.synchronize()
+        for i, x in enumerate(data):
+            _, w, P, F, S = oracle.ffa_search(x, tsamp, period_min=0.08, period_max=2.0,
+                                              bins_min=int(bmin[i]), bins_max=int(bmax[i]),
+                                              rmed_width=1.0)
+            Pb, Fb = b.grid(i)
+            assert np.array_equal(Pb, P) and np.array_equal(Fb, F)
+            np.testing.assert_allclose(_lib.to_numpy(outs[i]), S, rtol=1e-5, atol=1e-4)
+    b.close()

@@ -1039,13 +1039,19 @@ __device__ __forceinline__ bool tree_node(int m, int d, int i, int* r0, int* nm)
     return true;
 }
 
-// Next work item of a transform for the persistent kernel, in pass order:
-// pass 0 = whole nodes of depth depth[0] into buf_a, pass p >= 1 = cone tiles
-// of the nodes of depth depth[p], alternating buf_a -> buf_b -> buf_a ...,
-// the last pass producing S/N.  (p, i, lo) is the cursor; (node, r0, nm)
-// caches the rows of node `node` of pass p (nm = 0: a slot under a leaf).
+// Next work item of a transform for the persistent kernel: pass 0 = whole
+// nodes of depth depth[0] into buf_a, pass p >= 1 = cone tiles of the nodes
+// of depth depth[p], alternating buf_a -> buf_b -> buf_a ..., the last pass
+// producing S/N.  Three-pass transforms go depth first through passes 0 and
+// 1: the leaves under node j of depth depth[1], then node j's tiles, then
+// node j + 1, so that the tiles read rows this workgroup has just written
+// (about 1/2^depth[1] of the transform apart, instead of the whole pass)
+// while they are still in the caches; then the last pass.  (p, i, lo, j) is
+// the cursor (j: node of depth depth[1] of a three-pass transform); (node,
+// r0, nm) caches the rows of node `node` of pass p (nm = 0: a slot under a
+// leaf).
 __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int32_t* depth,
-                                              int& p, int& i, int& lo, int& node, int& r0,
+                                              int& p, int& i, int& lo, int& j, int& node, int& r0,
                                               int& nm, float* buf_a, float* buf_b, FfaTask& tk)
 {
     for (;;) {
@@ -1063,9 +1069,24 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
             return true;
         }
         const int d = depth[p];
-        if (i >= (1 << d)) {
-            ++p;
-            i = 0;
+        const bool dfs = tr.npass == 3 && p < 2;
+        // nodes of this pass visited now: all of them, or those under node j
+        const int i_end = dfs ? (j + 1) << (d - depth[1]) : 1 << d;
+        if (i >= i_end) {
+            if (!dfs) {
+                ++p;
+                i = 0;
+            } else if (p == 0) {
+                p = 1;  // node j's tiles
+                i = j;
+            } else if (j + 1 < (1 << depth[1])) {
+                ++j;  // the leaves under the next node
+                p = 0;
+                i = j << (depth[0] - depth[1]);
+            } else {
+                p = 2;  // the last pass
+                i = 0;
+            }
             lo = 0;
             node = -1;
             continue;
@@ -1147,11 +1168,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) 
         tk.rows_eval = tr.rows_eval;
         tk.snr = tr.snr;
         tk.pad = tr.cap;
-        int p = 0, i = 0, lo = 0, node = -1, r0 = 0, nm = 0;
+        int p = 0, i = 0, lo = 0, j = 0, node = -1, r0 = 0, nm = 0;
         // the pass depths are indexed dynamically: read them from memory, so
         // that the descriptor itself stays in registers
         const int32_t* depth = trs[item].depth;
-        while (next_ffa_task(tr, depth, p, i, lo, node, r0, nm, buf_a, buf_b, tk))
+        while (next_ffa_task(tr, depth, p, i, lo, j, node, r0, nm, buf_a, buf_b, tk))
             run_ffa_task<NT, STAGE, V>(tk, lds, err);
     }
 }

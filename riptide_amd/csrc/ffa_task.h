@@ -744,10 +744,12 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         for (int e = tid; e < nrows * nw; e += NT) mx[e] = ordered_int(-INFINITY);
         // widths and (h + b, b) pairs were copied to LDS at the start of
         // the task
+        // one LDS read per lane (lane j: width j), then the widths from the
+        // lanes, so that the reads are not serialised
+        const int w_lane = s_w[min(lane, nw - 1)];
         int w[kSnrMaxWidths];
 #pragma unroll
-        for (int j = 0; j < kSnrMaxWidths; ++j)
-            w[j] = __builtin_amdgcn_readfirstlane(s_w[j < nw ? j : nw - 1]);
+        for (int j = 0; j < kSnrMaxWidths; ++j) w[j] = __builtin_amdgcn_readlane(w_lane, j);
         // the widest window (the widths need not be in order)
         int wmax = w[0];
 #pragma unroll

@@ -465,17 +465,25 @@ struct RowCore {
     // of the final level, or -1 if the rows exceed the register staging
     // (tasks are sized so that they do not).  The row tables are built by
     // `tables` while this wavefront's row loads are in flight.
-    template <class Tables>
+    template <class Tables, class Setup>
     static __device__ __forceinline__ int run(const FfaTask& tk, const Seg* seg, float* data,
                                               const uint32_t* tab, uint32_t grow_v, int rows,
-                                              int pitch, int wave, int lane, Tables&& tables)
+                                              int pitch, int wave, int lane, Tables&& tables,
+                                              Setup&& setup)
     {
         const int bins = __builtin_amdgcn_readfirstlane(tk.bins);
-        if (rows > R * NW) return -1;
+        if (rows > R * NW) {
+            setup();
+            return -1;
+        }
         const int nr = (rows - wave + NW - 1) / NW;
         float v[R * K];
         load_issue(v, as_global(tk.src), tk.cone != 0, grow_v,
                    __builtin_amdgcn_readfirstlane(tk.node_r0), nr, bins, wave, lane);
+        // the merge coefficients of a whole-node task (one wavefront, one
+        // barrier) while its loads are in flight; a cone task needed them
+        // before its loads
+        setup();
         tables();
         load_commit(v, data, nr, bins, pitch, wave, lane);
         __syncthreads();
@@ -498,13 +506,14 @@ struct RowCore {
 // k = 2V and 2V+1; the wide variant one instantiation per rows-per-wavefront
 // class for k > kFfaMaxExactK.  Returns -1 for a k the variant does not hold
 // (the host groups work by variant, so this does not happen).
-template <int NW, int STAGE, int V, class Tables>
+template <int NW, int STAGE, int V, class Tables, class Setup>
 __device__ __forceinline__ int run_rows(const FfaTask& tk, const Seg* seg, float* data,
                                         const uint32_t* tab, uint32_t grow_v, int rows,
-                                        int pitch, int wave, int lane, Tables&& tables)
+                                        int pitch, int wave, int lane, Tables&& tables,
+                                        Setup&& setup)
 {
     const int k = (tk.bins + kWave - 1) / kWave;
-#define RT_CORE_ARGS tk, seg, data, tab, grow_v, rows, pitch, wave, lane, tables
+#define RT_CORE_ARGS tk, seg, data, tab, grow_v, rows, pitch, wave, lane, tables, setup
     if constexpr (V < kFfaWideVariant) {
         constexpr int KA = 2 * V, KB = 2 * V + 1;
         static_assert(KB <= kFfaMaxExactK, "variant beyond the exact column counts");
@@ -937,16 +946,21 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     __shared__ int s_w[kSnrMaxWidths];
     __shared__ float s_hb[2 * kSnrMaxWidths];
 
-    // ---- merge coefficients and the cone (one wavefront)
-    if (wave == 0) {
-        fill_merge_coef(&s_coef, M, levels, lane);
-        if (tk.cone) {
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            build_cone_wave(seg, tk, &s_coef, &s_bottom, lane);
+    // ---- merge coefficients and the cone (one wavefront, one barrier):
+    // before the row loads of a cone task (they need the cone), while they
+    // are in flight for a whole-node task
+    auto setup = [&]() {
+        if (wave == 0) {
+            fill_merge_coef(&s_coef, M, levels, lane);
+            if (tk.cone) {
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                build_cone_wave(seg, tk, &s_coef, &s_bottom, lane);
+            }
         }
-    }
-    __syncthreads();
+        __syncthreads();
+    };
+    if (tk.cone) setup();
     int rows;  // rows in the current level's layout
     uint32_t grow_v = 0;
     if (tk.cone) {
@@ -988,7 +1002,10 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
 
     // ---- staging (tables meanwhile), merge levels (and the store of kind-0
     // tasks)
-    rows = run_rows<NW, STAGE, V>(tk, seg, data, tab, grow_v, rows, pitch, wave, lane, tables);
+    rows = run_rows<NW, STAGE, V>(tk, seg, data, tab, grow_v, rows, pitch, wave, lane, tables,
+                                  [&]() {
+                                      if (!tk.cone) setup();
+                                  });
     if (rows < 0) {
         // a row length this kernel variant was not built for
         if (tid == 0) atomicOr(err, 2);

@@ -379,6 +379,7 @@ int rt_time_ffa(int64_t rows, int64_t cols, int shape, int with_snr, int64_t rep
     sp.nw = (int)w.size();
     sp.stdnoise = 1.0f;
     sp.rows_eval = (int)rows;
+    sp.wmax = w.empty() ? 0 : (int)w.back();
     std::vector<std::vector<FfaTask>> passes;
     if (plan_transform_tasks(sp, passes) < 0) return fail(RT_EINVAL, "cannot tile this block");
     std::vector<FfaTask> all;
@@ -659,6 +660,7 @@ struct PgramSetup {
     int32_t* d_w = nullptr;
     float* d_ds = nullptr;
     int64_t nw = 0;
+    int wmax = 0;
 };
 
 // Plans the search, checks the widths, uploads the period grid and the S/N
@@ -675,6 +677,8 @@ static int pgram_setup(const float* d_in, int64_t size, double tsamp, const uint
     int st = check_widths(h_widths, nw, bmin);
     if (st) return st;
     out->nw = nw;
+    out->wmax = 0;
+    for (int64_t i = 0; i < nw; ++i) out->wmax = std::max<int>(out->wmax, (int)h_widths[i]);
 
     std::vector<double> periods((size_t)p.length);
     std::vector<uint32_t> foldbins((size_t)p.length);
@@ -760,8 +764,10 @@ static TransformSpec pgram_spec(const PgramSetup& su, const float* d_in, const S
     sp.m = t.rows;
     sp.bins = t.bins;
     sp.cap = ffa_cap_rows(t.bins, su.shape);
+    sp.shape = su.shape;
     sp.src = (stp.ds_offset < 0) ? d_in : su.d_ds + stp.ds_offset;
     sp.kind = 1;
+    sp.wmax = su.wmax;
     sp.snr = d_snrs + t.out_row * su.nw;
     sp.widths = su.d_w;
     sp.hb = su.d_hb + (size_t)(t.bins - p.bmin) * su.nw * 2;

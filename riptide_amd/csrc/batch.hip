@@ -346,18 +346,24 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                 f.cap = tcap;
                 f.shape = tshape;
                 f.series = s;
+                f.wmax = 0;
+                for (uint64_t x : b->wlists[wg]) f.wmax = std::max<int32_t>(f.wmax, (int32_t)x);
+                f.ext_rows = ffa_snr_ext_rows(t.bins, f.wmax, f.nw, tshape);
+                f.k_last = 0;
                 if (t.rows <= f.cap) {
                     f.npass = 1;
                     f.levels0 = ffa_height(t.rows);
                 } else {
-                    const int np = count_transform_passes(t.rows, t.bins, f.cap, tlmax);
+                    const int np = count_transform_passes(t.rows, t.bins, f.cap, tlmax, f.ext_rows);
                     if (np < 0 || np > kMaxPasses) {
                         free_batch(b);
                         return bfail(RT_EINVAL, "bins too large for the FFA tiling");
                     }
-                    std::vector<int> depths = transform_pass_depths(t.rows, f.cap, tlmax);
+                    std::vector<int> depths = transform_pass_depths(t.rows, f.cap, tlmax, f.ext_rows);
                     f.npass = (int32_t)depths.size();
                     for (size_t q = 0; q < depths.size(); ++q) f.depth[q] = depths[q];
+                    f.k_last = (int32_t)last_tile_rows(
+                        f.cap, depths[depths.size() - 2] - depths[depths.size() - 1], f.ext_rows);
                     max_fold = std::max<int64_t>(max_fold, (int64_t)t.rows * t.bins);
                 }
                 order.emplace_back(transform_cost(t), (int)local.size());

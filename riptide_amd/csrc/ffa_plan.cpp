@@ -57,9 +57,24 @@ static int leaf_depth(int64_t m, int cap)
 // internal segment grows by at most 2 rows per level.
 static int64_t tile_rows(int cap, int L) { return (int64_t)cap - 2 * (((int64_t)1 << L) - 1); }
 
+int ffa_snr_ext_rows(int bins, int wmax, int nw, int shape)
+{
+    if (wmax < 3 || nw < 1 || nw > kSnrMaxWidths || wmax >= bins) return 0;
+    // finished rows start one row (the -0.0f row) into the dynamic LDS
+    const int avail = kFfaShape[shape].lds_bytes / 4 - ffa_pitch(bins);
+    return std::max(0, avail / (ffa_snr_pitch(bins, wmax) + nw));
+}
+
+int64_t last_tile_rows(int cap, int L, int ext_rows)
+{
+    const int64_t K = tile_rows(cap, L);
+    if (ext_rows > 0 && ext_rows < K && 8 * (int64_t)ext_rows >= 7 * K) return ext_rows;
+    return K;
+}
+
 // Chooses the depths d_0 = d1 > d_1 > ... > d_P = 0 of the cone passes, at
-// most lmax levels per pass.
-static std::vector<int> cone_depths(int d1, int cap, int lmax)
+// most lmax levels per pass (the last pass's tiles as last_tile_rows).
+static std::vector<int> cone_depths(int d1, int cap, int lmax, int ext_rows)
 {
     std::vector<int> best;
     double best_cost = 1e300;
@@ -72,7 +87,7 @@ static std::vector<int> cone_depths(int d1, int cap, int lmax)
         for (int p = 0; p < P; ++p) {
             const int L = (rem + (P - p) - 1) / (P - p);
             if (L > lmax || L > kMaxConeLevels) { ok = false; break; }
-            const int64_t K = tile_rows(cap, L);
+            const int64_t K = p == P - 1 ? last_tile_rows(cap, L, ext_rows) : tile_rows(cap, L);
             if (K < 4) { ok = false; break; }
             cost += (double)(K + 2 * ((1 << L) - 1)) / (double)K;  // reads per output row
             rem -= L;
@@ -88,10 +103,10 @@ static std::vector<int> cone_depths(int d1, int cap, int lmax)
     return best;
 }
 
-std::vector<int> transform_pass_depths(int64_t m, int cap, int lmax)
+std::vector<int> transform_pass_depths(int64_t m, int cap, int lmax, int ext_rows)
 {
     if (m <= cap) return {0};
-    return cone_depths(leaf_depth(m, cap), cap, lmax);
+    return cone_depths(leaf_depth(m, cap), cap, lmax, ext_rows);
 }
 
 int64_t ffa_adds(int64_t m)
@@ -101,12 +116,12 @@ int64_t ffa_adds(int64_t m)
     return ffa_adds(mh) + ffa_adds(m - mh) + m;
 }
 
-int count_transform_passes(int64_t m, int bins, int cap, int lmax)
+int count_transform_passes(int64_t m, int bins, int cap, int lmax, int ext_rows)
 {
     (void)bins;
     if (m <= cap) return 1;
     const int d1 = leaf_depth(m, cap);
-    const std::vector<int> depths = cone_depths(d1, cap, lmax);
+    const std::vector<int> depths = cone_depths(d1, cap, lmax, ext_rows);
     if (depths.empty()) return -1;
     return (int)depths.size();  // leaf pass + (size-1) cone passes
 }
@@ -122,6 +137,8 @@ static FfaTask base_task(const TransformSpec& s)
     t.rows_eval = s.rows_eval;
     t.snr = s.snr;
     t.pad = s.cap;
+    t.wmax = s.kind == 1 ? s.wmax : 0;
+    t.ext_rows = s.kind == 1 ? ffa_snr_ext_rows(s.bins, s.wmax, s.nw, s.shape) : 0;
     return t;
 }
 
@@ -146,7 +163,8 @@ int plan_transform_tasks(const TransformSpec& s, std::vector<std::vector<FfaTask
         return 1;
     }
     const int d1 = leaf_depth(s.m, s.cap);
-    const std::vector<int> depths = cone_depths(d1, s.cap, kMaxConeLevels);
+    const int ext_rows = s.kind == 1 ? ffa_snr_ext_rows(s.bins, s.wmax, s.nw, s.shape) : 0;
+    const std::vector<int> depths = cone_depths(d1, s.cap, kMaxConeLevels, ext_rows);
     if (depths.empty()) return -1;
     const int npass = (int)depths.size();
     ensure((size_t)npass);
@@ -174,8 +192,8 @@ int plan_transform_tasks(const TransformSpec& s, std::vector<std::vector<FfaTask
         const int d_in = depths[p - 1];
         const int d_out = depths[p];
         const int L = d_in - d_out;
-        const int64_t K = tile_rows(s.cap, L);
         const bool last = (p == npass - 1);
+        const int64_t K = last ? last_tile_rows(s.cap, L, ext_rows) : tile_rows(s.cap, L);
         nodes.clear();
         nodes_at_depth(0, s.m, d_out, nodes);
         for (const auto& nd : nodes) {

@@ -43,7 +43,19 @@ struct TransformSpec {
     int nw = 0;
     float stdnoise = 1.0f;
     int rows_eval = 0;
+    int wmax = 0;           // widest trial width (kind 1; 0: no prefix extension)
 };
+
+// Most rows of a finished level whose circular prefix sums, extended by
+// wmax values per row, fit the LDS of workgroup shape `shape` together with
+// the per-width window maxima of those rows; 0 when the extension is not
+// used (no S/N, more widths than the register path holds, or wmax < 3).
+int ffa_snr_ext_rows(int bins, int wmax, int nw, int shape);
+
+// Tile rows of the last (S/N) cone pass with L merge levels: the tile of
+// tile_rows(cap, L) rows, cut to ext_rows where that costs at most 1/8 of
+// the tile so that every tile's prefix sums take the extension.
+int64_t last_tile_rows(int cap, int L, int ext_rows);
 
 // Appends the transform's tasks to passes[0..], growing `passes` as needed;
 // returns the number of passes used, or -1 if the transform cannot be tiled
@@ -54,12 +66,14 @@ int plan_transform_tasks(const TransformSpec& s, std::vector<std::vector<FfaTask
 // whole nodes of depth d1, pass p the nodes of depth depths[p]); {0} when the
 // transform fits one workgroup; empty when it cannot be tiled.  At most lmax
 // merge levels per cone pass.
-std::vector<int> transform_pass_depths(int64_t m, int cap, int lmax = kMaxConeLevels);
+std::vector<int> transform_pass_depths(int64_t m, int cap, int lmax = kMaxConeLevels,
+                                       int ext_rows = 0);
 
 // Row additions of an m-row FFA (multiply by the bins for float adds).
 int64_t ffa_adds(int64_t m);
 
 // Number of passes plan_transform_tasks would use (no tasks built).
-int count_transform_passes(int64_t m, int bins, int cap, int lmax = kMaxConeLevels);
+int count_transform_passes(int64_t m, int bins, int cap, int lmax = kMaxConeLevels,
+                           int ext_rows = 0);
 
 }  // namespace rt

@@ -24,6 +24,50 @@
 
 namespace rt {
 
+// Device-clock phase accounting of the FFA kernels, compiled in only with
+// -DRT_PHASE_TIMING: thread 0 of a workgroup charges the cycles since its
+// previous mark to phase `id` (0 queue and task iteration, 1 merge
+// coefficients and cone, 2 staging and row tables, 3 / 4 merge levels of
+// whole-node / cone tasks, 5 row store, 6 S/N prefix sums, 7 window maxima,
+// 8 S/N writes); rt_phase_read adds up every workgroup's totals.
+#ifdef RT_PHASE_TIMING
+constexpr int kPhases = 16;
+static __device__ unsigned long long g_rt_phase[kPhases];
+__device__ __forceinline__ unsigned long long* rt_phase_lds()
+{
+    __shared__ unsigned long long s[kPhases + 1];
+    return s;
+}
+#define RT_PHASE_START()                                                    \
+    do {                                                                    \
+        if (threadIdx.x == 0) {                                             \
+            unsigned long long* s_ = rt_phase_lds();                        \
+            for (int i_ = 0; i_ < kPhases; ++i_) s_[i_] = 0;                \
+            s_[kPhases] = clock64();                                        \
+        }                                                                   \
+    } while (0)
+#define RT_PHASE(id)                                                        \
+    do {                                                                    \
+        if (threadIdx.x == 0) {                                             \
+            unsigned long long* s_ = rt_phase_lds();                        \
+            const unsigned long long t_ = clock64();                        \
+            s_[(id)] += t_ - s_[kPhases];                                   \
+            s_[kPhases] = t_;                                               \
+        }                                                                   \
+    } while (0)
+#define RT_PHASE_END()                                                      \
+    do {                                                                    \
+        if (threadIdx.x == 0) {                                             \
+            unsigned long long* s_ = rt_phase_lds();                        \
+            for (int i_ = 0; i_ < kPhases; ++i_) atomicAdd(&g_rt_phase[i_], s_[i_]); \
+        }                                                                   \
+    } while (0)
+#else
+#define RT_PHASE_START() do { } while (0)
+#define RT_PHASE(id) do { } while (0)
+#define RT_PHASE_END() do { } while (0)
+#endif
+
 // LDS layout of the FFA task kernel (bytes are dynamic; see ffa_lds_bytes()):
 //   float    zero[bins]                       all -0.0f (tail of a copied row)
 //   float    data[cap_rows * bins]            staged rows
@@ -311,8 +355,8 @@ struct RowCore {
     // traffic.  Row -1 of `data` is the -0.0f row, the tail of copied rows.
     static __device__ __forceinline__ int merge(const Seg* seg, float* data, const uint32_t* tab,
                                                 int rows, int levels, bool cone, int bins,
-                                                int pitch, int cap_rows, int wave, int lane,
-                                                gfloat* gdst)
+                                                int pitch, int last_pitch, int cap_rows, int wave,
+                                                int lane, gfloat* gdst)
     {
         const int kd = (bins + kWave - 1) / kWave;
         const bool last_ok = lane + (K - 1) * kWave < bins;
@@ -445,10 +489,13 @@ struct RowCore {
                 }
                 return out_rows;
             }
+            // the last level may be laid out at another pitch (rows with
+            // room for their S/N prefix extension)
+            const int opitch = to == 0 ? last_pitch : pitch;
 #pragma unroll
             for (int i = 0; i < R; ++i) {
                 if (i < nr) {
-                    float* orow = data + (wave + i * NW) * pitch + lane;
+                    float* orow = data + (wave + i * NW) * opitch + lane;
 #pragma unroll
                     for (int q = 0; q < K; ++q)
                         if (col_ok(q, kd, last_ok, lane, bins)) orow[q * kWave] = v[i * K + q];
@@ -468,8 +515,8 @@ struct RowCore {
     template <class Tables, class Setup>
     static __device__ __forceinline__ int run(const FfaTask& tk, const Seg* seg, float* data,
                                               const uint32_t* tab, uint32_t grow_v, int rows,
-                                              int pitch, int wave, int lane, Tables&& tables,
-                                              Setup&& setup)
+                                              int pitch, int last_pitch, int wave, int lane,
+                                              Tables&& tables, Setup&& setup)
     {
         const int bins = __builtin_amdgcn_readfirstlane(tk.bins);
         if (rows > R * NW) {
@@ -487,15 +534,18 @@ struct RowCore {
         tables();
         load_commit(v, data, nr, bins, pitch, wave, lane);
         __syncthreads();
+        RT_PHASE(2);
         const int levels = __builtin_amdgcn_readfirstlane(tk.levels);
         gfloat* dst = tk.kind == 0 ? as_global(tk.dst) + ((size_t)tk.node_r0 + tk.lo) * bins : nullptr;
-        rows = merge(seg, data, tab, rows, levels, tk.cone != 0, bins, pitch,
+        rows = merge(seg, data, tab, rows, levels, tk.cone != 0, bins, pitch, last_pitch,
                      __builtin_amdgcn_readfirstlane(tk.pad), wave, lane, dst);
+        RT_PHASE(tk.cone ? 4 : 3);
         if (tk.kind == 0 && (levels == 0 || K < 6)) {
             // no merge level (the staged rows themselves), or the last level
             // was written back to LDS
             store(dst, data, rows, bins, pitch, wave, lane);
             __syncthreads();
+            RT_PHASE(5);
         }
         return rows;
     }
@@ -509,11 +559,11 @@ struct RowCore {
 template <int NW, int STAGE, int V, class Tables, class Setup>
 __device__ __forceinline__ int run_rows(const FfaTask& tk, const Seg* seg, float* data,
                                         const uint32_t* tab, uint32_t grow_v, int rows,
-                                        int pitch, int wave, int lane, Tables&& tables,
-                                        Setup&& setup)
+                                        int pitch, int last_pitch, int wave, int lane,
+                                        Tables&& tables, Setup&& setup)
 {
     const int k = (tk.bins + kWave - 1) / kWave;
-#define RT_CORE_ARGS tk, seg, data, tab, grow_v, rows, pitch, wave, lane, tables, setup
+#define RT_CORE_ARGS tk, seg, data, tab, grow_v, rows, pitch, last_pitch, wave, lane, tables, setup
     if constexpr (V < kFfaWideVariant) {
         constexpr int KA = 2 * V, KB = 2 * V + 1;
         static_assert(KB <= kFfaMaxExactK, "variant beyond the exact column counts");
@@ -658,7 +708,7 @@ __device__ __forceinline__ void window_max_default(const float* cps, int i0, int
 template <int NT>
 __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pitch, int nrows,
                                          float* scratch, int scratch_floats, int* s_w,
-                                         float* s_hb)
+                                         float* s_hb, int ext_w)
 {
     constexpr int NW = NT / kWave;
     const int tid = threadIdx.x;
@@ -673,12 +723,24 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
     // columns (the last chunk also takes the remainder); the lanes of one
     // wavefront work on consecutive rows of one or two chunks.
     int C = max(1, NT / nrows);
-    C = min(C, max(1, bins / 8));                            // chunks of >= 8 columns
-    C = min(C, max(1, (scratch_floats - 1) / (2 * nrows)));  // chunk totals fit
+    C = min(C, max(1, bins / 8));  // chunks of >= 8 columns
+    // float64 chunk totals: [C][nrows] in the scratch, or (rows with room
+    // for the prefix extension, ext_w > 0) the C totals of row p in that
+    // row's extension, which the extension overwrites once they are used
+    if (ext_w)
+        C = min(C, max(1, (ext_w - 1) / 2));
+    else
+        C = min(C, max(1, (scratch_floats - 1) / (2 * nrows)));
     const int nitems = C * nrows;
-    // [C][nrows] float64 chunk totals, 8-byte aligned
     double* tot = reinterpret_cast<double*>((reinterpret_cast<uintptr_t>(scratch) + 7) &
                                             ~static_cast<uintptr_t>(7));
+    auto tot_at = [&](int c, int p) -> double* {
+        if (ext_w) {
+            const uintptr_t e = reinterpret_cast<uintptr_t>(data + (size_t)p * pitch + bins);
+            return reinterpret_cast<double*>((e + 7) & ~static_cast<uintptr_t>(7)) + c;
+        }
+        return tot + (size_t)c * nrows + p;
+    };
     // widths and (h + b, b) pairs of the transform: loaded now, stored to
     // LDS after the first prefix pass, so that their latency is hidden
     const int nwl = min(nw, kSnrMaxWidths);
@@ -706,7 +768,7 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             for (int u = 0; u < 4; ++u) acc += (double)x[u];
         }
         for (; j < je; ++j) acc += (double)row[j];
-        tot[it] = acc;
+        *tot_at(c, p) = acc;
     }
     if (wave == NW - 1) {
         if (lane < nwl) s_w[lane] = w_ld;
@@ -722,7 +784,7 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         const int c = it / nrows;
         const int p = it - c * nrows;
         double acc = 0.0;
-        for (int q = 0; q < c; ++q) acc += tot[q * nrows + p];
+        for (int q = 0; q < c; ++q) acc += *tot_at(q, p);
         float* row = data + (size_t)p * pitch;
         int j = 4 * ((c * g4) / C);
         const int je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
@@ -745,6 +807,7 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         }
     }
     __syncthreads();
+    RT_PHASE(6);
 
     gcfloat* hb = as_global(tk.hb);
     gcint* gw = as_global(tk.widths);
@@ -768,11 +831,23 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
 #pragma unroll
         for (int j = 0; j < kSnrMaxWidths; ++j)
             if (j < nw && w[j] != kDefaultWidths[j]) dflt = false;
+        if (ext_w) {
+            // the circular extension of the prefix sums, cps[bins + j] =
+            // cps[j] + total for j < wmax (kernels.hpp:86-101, float32), over
+            // the chunk totals, which are no longer read
+            for (int e = tid; e < nrows * ext_w; e += NT) {
+                const int p = e / ext_w;
+                const int j = e - p * ext_w;
+                float* row = data + (size_t)p * pitch;
+                row[bins + j] = row[j] + row[bins - 1];
+            }
+        }
         __syncthreads();
-        // (a) window starts [0, bins - wmax), whose windows all stay inside
-        // the row, cut into the same C chunks of whole groups of 4, all
-        // widths per thread
-        const int P = max(0, bins - wmax);
+        // (a) window starts [0, P) whose windows all stay inside the row (or
+        // its extension), cut into the same C chunks of whole groups of 4,
+        // all widths per thread: every start when the rows carry the
+        // extension, else [0, bins - wmax)
+        const int P = ext_w ? bins : max(0, bins - wmax);
         const int q4 = P >> 2;
         for (int it = tid; it < nitems; it += NT) {
             const int c = it / nrows;
@@ -806,14 +881,14 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             for (int j = 0; j < kSnrMaxWidths; ++j)
                 if (j < nw) atomicMax(&mx[p * nw + j], ordered_int(m[j]));
         }
-        // (b) the last wmax window starts [P, bins), one thread per (row,
-        // width), width-major.  The window at k ends inside the row while k <
+        // (b) without the extension, the last wmax window starts [P, bins),
+        // one thread per (row, width), width-major.  The window at k ends inside the row while k <
         // split = bins - wj, else on the extension cps[j] + total (kernels.hpp:
         // 88-100).  Eight starts per step with all sixteen loads issued
         // before the first use; the step that crosses split and the last
         // partial step take the per-start form (starts past the row clamped
         // to its last start, which leaves the maximum unchanged).
-        for (int job = tid; job < nrows * nw; job += NT) {
+        for (int job = tid; job < (ext_w ? 0 : nrows * nw); job += NT) {
             const int iw = job / nrows;
             const int p = job - iw * nrows;
             const float* cps = data + (size_t)p * pitch;
@@ -869,6 +944,7 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             atomicMax(&mx[p * nw + iw], ordered_int(d));
         }
         __syncthreads();
+        RT_PHASE(7);
         for (int e = tid; e < nrows * nw; e += NT) {
             const int p = e / nw;
             const int iw = e - p * nw;
@@ -879,6 +955,7 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
             as_global(tk.snr)[(size_t)(tk.lo + p) * nw + iw] = snr;
         }
         __syncthreads();
+        RT_PHASE(8);
         return;
     }
     // many widths: one thread per (row, width), width-major (uniform loop
@@ -961,6 +1038,7 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
         __syncthreads();
     };
     if (tk.cone) setup();
+    RT_PHASE(1);
     int rows;  // rows in the current level's layout
     uint32_t grow_v = 0;
     if (tk.cone) {
@@ -1002,8 +1080,24 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
 
     // ---- staging (tables meanwhile), merge levels (and the store of kind-0
     // tasks)
-    rows = run_rows<NW, STAGE, V>(tk, seg, data, tab, grow_v, rows, pitch, wave, lane, tables,
-                                  [&]() {
+    // S/N tasks whose last merge level fits in LDS with the circular
+    // extension of its prefix sums: that level is written at the extended
+    // pitch, so that every window start reads its window in place
+    constexpr int kLdsFloats = kFfaShape[ffa_shape_of(NT, STAGE)].lds_bytes / 4;
+    int spitch = pitch;
+    int ext_w = 0;
+    if (tk.kind == 1 && levels > 0) {
+        const int wmax = __builtin_amdgcn_readfirstlane(tk.wmax);
+        const int final_rows = tk.cone ? tk.hi - tk.lo : M;
+        const int p2 = ffa_snr_pitch(bins, wmax);
+        if (wmax >= 3 && wmax < bins && final_rows <= tk.ext_rows &&
+            final_rows * (p2 + tk.nw) <= kLdsFloats - pitch) {
+            spitch = p2;
+            ext_w = wmax;
+        }
+    }
+    rows = run_rows<NW, STAGE, V>(tk, seg, data, tab, grow_v, rows, pitch, spitch, wave, lane,
+                                  tables, [&]() {
                                       if (!tk.cone) setup();
                                   });
     if (rows < 0) {
@@ -1018,10 +1112,11 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     const int nrows = max(0, s_end - tk.lo);
     // LDS past the S/N rows is free: the other staged rows, the row tables
     // and the cone segments
-    float* scratch = data + (size_t)nrows * pitch;
+    float* scratch = data + (size_t)nrows * spitch;
     const int scratch_floats =
-        (cap_rows - nrows) * pitch + kFfaMaxLevels * cap_rows + (2 << kMaxConeLevels) * 5;
-    snr_rows<NT>(tk, data, pitch, nrows, scratch, scratch_floats, s_w, s_hb);
+        ext_w ? kLdsFloats - pitch - nrows * spitch
+              : (cap_rows - nrows) * pitch + kFfaMaxLevels * cap_rows + (2 << kMaxConeLevels) * 5;
+    snr_rows<NT>(tk, data, spitch, nrows, scratch, scratch_floats, s_w, s_hb, ext_w);
 }
 
 // One workgroup per task (the task list of one pass).
@@ -1135,8 +1230,8 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
             return true;
         }
         const int L = depth[p - 1] - d;
-        const int K = tr.cap - 2 * ((1 << L) - 1);
         const bool last = (p == tr.npass - 1);
+        const int K = last ? tr.k_last : tr.cap - 2 * ((1 << L) - 1);
         if (lo >= nm || (last && lo >= tr.rows_eval)) {
             ++i;
             lo = 0;
@@ -1170,13 +1265,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) 
     __shared__ int s_item;
     float* buf_a = scratch + (size_t)blockIdx.x * slot_floats;
     float* buf_b = buf_a + slot_floats / 2;
+    RT_PHASE_START();
     for (;;) {
         if (threadIdx.x == 0) s_item = atomicAdd(queue, 1);
         __syncthreads();
         // uniform across the workgroup: keep the descriptor in scalar registers
         const int item = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
-        if (item >= ntr) return;
+        if (item >= ntr) {
+            RT_PHASE(0);
+            RT_PHASE_END();
+            return;
+        }
         const FfaTransform tr = trs[item];
         FfaTask tk = {};
         tk.bins = tr.bins;
@@ -1187,12 +1287,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) 
         tk.rows_eval = tr.rows_eval;
         tk.snr = tr.snr;
         tk.pad = tr.cap;
+        tk.wmax = tr.wmax;
+        tk.ext_rows = tr.ext_rows;
         int p = 0, i = 0, lo = 0, j = 0, node = -1, r0 = 0, nm = 0;
         // the pass depths are indexed dynamically: read them from memory, so
         // that the descriptor itself stays in registers
         const int32_t* depth = trs[item].depth;
-        while (next_ffa_task(tr, depth, p, i, lo, j, node, r0, nm, buf_a, buf_b, tk))
+        while (next_ffa_task(tr, depth, p, i, lo, j, node, r0, nm, buf_a, buf_b, tk)) {
+            RT_PHASE(0);
             run_ffa_task<NT, STAGE, V>(tk, lds, err);
+        }
     }
 }
 

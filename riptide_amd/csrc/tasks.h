@@ -25,6 +25,12 @@ constexpr FfaShape kFfaShape[kFfaShapes] = {
 };
 // Largest stage over the shapes.
 constexpr int kStage = 72;
+// The shape of a kernel built with `threads` threads and `stage` staged
+// values per lane.
+constexpr int ffa_shape_of(int threads, int stage)
+{
+    return threads == 1024 ? 0 : (stage == 40 ? 1 : 2);
+}
 // Widths whose window maxima a thread of the S/N phase keeps in registers
 // (more widths take a slower path).
 constexpr int kSnrMaxWidths = 16;
@@ -32,6 +38,9 @@ constexpr int kSnrMaxWidths = 16;
 // LDS row pitch of rows of `bins` floats: odd, so that the lanes of a
 // wavefront reading one column of consecutive rows hit distinct banks.
 constexpr int ffa_pitch(int bins) { return bins | 1; }
+// LDS row pitch of finished rows whose circular prefix sums are extended by
+// the widest trial width wmax (snr.hpp:37-41: prefix over size + wmax).
+constexpr int ffa_snr_pitch(int bins, int wmax) { return (bins + wmax) | 1; }
 
 // Rows per wavefront per merge level with k columns per lane: what the
 // staging holds, at most 16 (narrow rows would otherwise need more row-table
@@ -103,6 +112,11 @@ struct FfaTask {
     int32_t nw;
     float stdnoise;
     int32_t pad;
+    // kind 1: widest trial width, and the most rows of the last merge level
+    // whose circular prefix sums fit in LDS with their extension of wmax
+    // values (rows at pitch ffa_snr_pitch); 0: no extension
+    int32_t wmax;
+    int32_t ext_rows;
 };
 
 // One whole FFA transform for the persistent batch kernel: the pass split
@@ -125,6 +139,9 @@ struct FfaTransform {
     int32_t depth[kMaxPasses];
     int32_t series;         // owning series (bookkeeping)
     int32_t shape;          // workgroup shape (kFfaShape index) the tiling assumes
+    int32_t wmax;           // as FfaTask::wmax / ext_rows
+    int32_t ext_rows;
+    int32_t k_last;         // tile rows of the last (S/N) cone pass
 };
 
 // One downsampling step of one series for the packed multi-step kernel.

@@ -252,8 +252,10 @@ int rt_batch_run_timed(void* handle, void* stream, double* ms);
 /* Waits for `stream` and reports a device-side tiling error, if any. */
 int rt_batch_check(void* handle, void* stream);
 
-/* stats[8]: transforms, groups, workgroup slots, scratch bytes, packed
- * downsampling bytes, folded input floats, trial periods, FFA float adds. */
+/* stats[10]: transforms, groups, workgroup slots, scratch bytes, packed
+ * downsampling bytes, folded input floats, trial periods, FFA float adds,
+ * tile records (the cones and row tables of transforms several series share,
+ * built once per run) and their bytes. */
 int rt_batch_stats(void* handle, int64_t* stats);
 
 int rt_batch_destroy(void* handle);

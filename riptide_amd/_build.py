@@ -15,8 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libriptide_amd.so")
 BUILD = os.path.join(HERE, "_build")
 
-HIP_SOURCES = ["ffa_kernels.hip", "series_kernels.hip", "batch.hip", "adjoint.hip", "api.hip",
-               "fir.hip", "peaks.hip", "grad_batch.hip"]
+HIP_SOURCES = ["ffa_kernels.hip", "ffa_records.hip", "series_kernels.hip", "batch.hip",
+               "adjoint.hip", "api.hip", "fir.hip", "peaks.hip", "grad_batch.hip"]
 GCC_SOURCES = ["plan.cpp", "ffa_plan.cpp"]
 # ffa_variant.hip is compiled once per FFA kernel variant (kFfaVariants in
 # csrc/tasks.h), each holding the merge code of its own row lengths

@@ -115,10 +115,10 @@ class BatchSearch:
         _lib.call("rt_batch_check", self.handle, _lib.stream())
 
     def stats(self):
-        s = np.zeros(8, np.int64)
+        s = np.zeros(10, np.int64)
         _lib.call("rt_batch_stats", self.handle, _lib.host_ptr(s))
         keys = ["transforms", "groups", "slots", "scratch_bytes", "ds_bytes", "fold_floats",
-                "periods", "ffa_adds"]
+                "periods", "ffa_adds", "tile_records", "tile_record_bytes"]
         return dict(zip(keys, (int(v) for v in s)))
 
     def close(self):

@@ -131,6 +131,9 @@ struct Batch {
     std::vector<FfaTransform> host_trs;  // all groups, src/snr filled at bind
     std::vector<int> tr_series;
     std::vector<int> tr_plan_index;      // transform index within its plan
+    std::vector<int64_t> tr_rec_off;     // per transform: word offset of its records, or -1
+    std::vector<FfaRecordSpec> rec_specs;  // tile records shared by equal transforms
+    int64_t rec_words_total = 0;
     std::vector<DsSeriesDev> host_ds;
     std::vector<DsStepDev> host_steps;
     std::vector<PrepSeriesDev> host_prep;  // per series, pointers filled at bind
@@ -140,6 +143,8 @@ struct Batch {
     FfaTransform* d_trs = nullptr;
     DsSeriesDev* d_ds_series = nullptr;
     DsStepDev* d_ds_steps = nullptr;
+    int32_t* d_rec = nullptr;
+    FfaRecordSpec* d_rec_specs = nullptr;
     PrepSeriesDev* d_prep_series = nullptr;
     float* d_ds = nullptr;
     float* d_prep = nullptr;
@@ -167,7 +172,7 @@ static void free_batch(Batch* b)
     if (b->graph_stream) hipStreamDestroy(b->graph_stream);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
-    for (void* p : {(void*)b->d_tables, (void*)b->d_trs, (void*)b->d_ds_series,
+    for (void* p : {(void*)b->d_rec, (void*)b->d_rec_specs, (void*)b->d_tables, (void*)b->d_trs, (void*)b->d_ds_series,
                     (void*)b->d_ds_steps, (void*)b->d_prep_series, (void*)b->d_ds,
                     (void*)b->d_prep,
                     (void*)b->d_scratch, (void*)b->d_work, (void*)b->d_queue, (void*)b->d_err})
@@ -178,6 +183,53 @@ static void free_batch(Batch* b)
 static int64_t transform_cost(const Transform& t)
 {
     return (int64_t)t.rows * t.bins * (ffa_height(t.rows) + 4);
+}
+
+// Tile records of one transform shape (FfaRecord layout in tasks.h): one per
+// (pass, node size class, tile) in the order next_ffa_task indexes them;
+// appends their specs to b->rec_specs and sets f->rec_first.  Returns the
+// word offset of the first record.
+static int64_t plan_records(Batch* b, FfaTransform* f)
+{
+    const int64_t stride = rec_words(f->cap);
+    const int64_t off0 = b->rec_words_total;
+    int32_t count = 0;
+    auto add = [&](int32_t m, int32_t lo, int32_t hi, int32_t levels, int32_t cone) {
+        FfaRecordSpec r = {};
+        r.off = b->rec_words_total;
+        r.bins = f->bins;
+        r.cap = f->cap;
+        r.node_m = m;
+        r.lo = lo;
+        r.hi = hi;
+        r.levels = levels;
+        r.cone = cone;
+        b->rec_specs.push_back(r);
+        b->rec_words_total += stride;
+        ++count;
+    };
+    std::memset(f->rec_first, 0, sizeof(f->rec_first));
+    if (f->npass == 1) {
+        add(f->rows, 0, f->rows, f->levels0, 0);
+        return off0;
+    }
+    for (int p = 0; p < f->npass; ++p) {
+        const int d = f->depth[p];
+        const int L = p > 0 ? f->depth[p - 1] - d : 0;
+        const bool last = p == f->npass - 1;
+        const int K = p == 0 ? 0 : last ? f->k_last : f->cap - 2 * ((1 << L) - 1);
+        for (int c = 0; c < 2; ++c) {
+            const int m = (f->rows >> d) + c;
+            f->rec_first[p][c] = count;
+            if (m < 1) continue;
+            if (p == 0) {
+                add(m, 0, m, ffa_height(m), 0);
+            } else {
+                for (int lo = 0; lo < m; lo += K) add(m, lo, std::min(lo + K, m), L, 1);
+            }
+        }
+    }
+    return off0;
 }
 
 static int scrunch_factor(int64_t width_samples, int64_t minpts)
@@ -281,6 +333,12 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
         }
     }
 
+    // series per plan: the transforms of a plan that several series share
+    // get tile records, built once per run and read by every one of them
+    std::vector<int> plan_series_count(b->plans.size(), 0);
+    for (int64_t i = 0; i < nseries; ++i) ++plan_series_count[b->plan_of[i]];
+    std::map<std::pair<int, int>, std::pair<int64_t, FfaTransform>> rec_of;  // (plan, tr) -> records
+
     // groups of series; per group: packed downsampling descriptors and the
     // transform queue sorted by decreasing cost
     if (group_size < 1) group_size = 64;
@@ -298,6 +356,7 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
         const int tr_base = (int)b->host_trs.size();
         std::vector<FfaTransform> local;
         std::vector<int> local_series, local_pidx;
+        std::vector<int64_t> local_rec;
         for (int s = g.s0; s < g.s1; ++s) {
             const SeriesPlan& p = b->plans[b->plan_of[s]];
             b->ds_off[s] = ds_run;
@@ -366,10 +425,23 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                         f.cap, depths[depths.size() - 2] - depths[depths.size() - 1], f.ext_rows);
                     max_fold = std::max<int64_t>(max_fold, (int64_t)t.rows * t.bins);
                 }
+                int64_t rec_off = -1;
+                if (plan_series_count[b->plan_of[s]] >= kRecMinSeries) {
+                    const auto key = std::make_pair(b->plan_of[s], (int)it);
+                    auto rt = rec_of.find(key);
+                    if (rt == rec_of.end()) {
+                        FfaTransform g = f;
+                        const int64_t off = plan_records(b, &g);
+                        rt = rec_of.emplace(key, std::make_pair(off, g)).first;
+                    }
+                    rec_off = rt->second.first;
+                    std::memcpy(f.rec_first, rt->second.second.rec_first, sizeof(f.rec_first));
+                }
                 order.emplace_back(transform_cost(t), (int)local.size());
                 local.push_back(f);
                 local_series.push_back(s);
                 local_pidx.push_back((int)it);
+                local_rec.push_back(rec_off);
             }
         }
         // by kernel variant and workgroup shape, then largest first
@@ -397,6 +469,7 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
             b->host_trs.push_back(local[o.second]);
             b->tr_series.push_back(local_series[o.second]);
             b->tr_plan_index.push_back(local_pidx[o.second]);
+            b->tr_rec_off.push_back(local_rec[o.second]);
         }
         g.tr0 = tr_base;
         g.ntr = (int)order.size();
@@ -458,6 +531,15 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
     BT_HIP(hipMalloc(&b->d_queue, std::max<size_t>(b->nqueues, 1) * sizeof(int)));
     BT_HIP(hipMalloc(&b->d_err, sizeof(int)));
     BT_HIP(hipMemset(b->d_err, 0, sizeof(int)));
+    if (!b->rec_specs.empty()) {
+        BT_HIP(hipMalloc(&b->d_rec, (size_t)b->rec_words_total * sizeof(int32_t)));
+        BT_HIP(hipMemset(b->d_rec, 0, (size_t)b->rec_words_total * sizeof(int32_t)));
+        BT_HIP(hipMalloc(&b->d_rec_specs, b->rec_specs.size() * sizeof(FfaRecordSpec)));
+        BT_HIP(hipMemcpy(b->d_rec_specs, b->rec_specs.data(),
+                         b->rec_specs.size() * sizeof(FfaRecordSpec), hipMemcpyHostToDevice));
+    }
+    for (size_t k = 0; k < b->host_trs.size(); ++k)
+        b->host_trs[k].rec = b->tr_rec_off[k] >= 0 ? b->d_rec + b->tr_rec_off[k] : nullptr;
     *out = b;
     return RT_OK;
 }
@@ -541,6 +623,9 @@ static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
     // node, so that the run can be captured into a graph)
     hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s, b->d_err, 1, 0);
     BT_HIP(hipGetLastError());
+    // the tile records of the transforms that several series share
+    if (!b->rec_specs.empty())
+        BT_HIP(launch_ffa_records(b->d_rec_specs, (int)b->rec_specs.size(), b->d_rec, s));
     for (size_t gi = 0; gi < b->groups.size(); ++gi) {
         const Group& g = b->groups[gi];
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 0], s));
@@ -667,7 +752,8 @@ int rt_batch_set_tiling(const int32_t* entries, int n)
 int rt_batch_stats(void* handle, int64_t* stats)
 {
     // [0] transforms, [1] groups, [2] slots, [3] scratch bytes, [4] ds bytes,
-    // [5] fold input floats (sum rows*bins), [6] total periods, [7] FFA adds
+    // [5] fold input floats (sum rows*bins), [6] total periods, [7] FFA adds,
+    // [8] tile records, [9] tile record bytes
     Batch* b = (Batch*)handle;
     int64_t fold = 0, periods = 0, adds = 0;
     for (int64_t i = 0; i < b->nseries; ++i) {
@@ -686,6 +772,8 @@ int rt_batch_stats(void* handle, int64_t* stats)
     stats[5] = fold;
     stats[6] = periods;
     stats[7] = adds;
+    stats[8] = (int64_t)b->rec_specs.size();
+    stats[9] = b->rec_words_total * 4;
     return RT_OK;
 }
 

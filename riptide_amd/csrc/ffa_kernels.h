@@ -31,6 +31,11 @@ hipError_t launch_ffa_batch(const FfaTransform* d_trs, int ntr, int* d_queue, fl
                             int64_t slot_floats, int slots, int* d_err, hipStream_t stream,
                             int shape, int variant);
 
+// Builds n tile records (FfaRecordSpec -> FfaRecord layout in `out`): the
+// cone and row tables a task of each spec would compute itself.
+hipError_t launch_ffa_records(const FfaRecordSpec* d_specs, int n, int32_t* d_out,
+                              hipStream_t stream);
+
 // Launchers of one variant (ffa_task.h, built by ffa_variant.hip).
 template <int V>
 hipError_t launch_ffa_tasks_variant(const FfaTask* d_tasks, int ntasks, int* d_err,

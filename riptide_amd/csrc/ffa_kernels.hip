@@ -11,6 +11,7 @@
 
 namespace rt {
 
+
 // Plain S/N of rows x cols profiles (python_bindings.cpp:121-160): one
 // wavefront per row, the row staged in LDS.
 __global__ __launch_bounds__(256) void snr_rows_kernel(const float* __restrict__ x, int rows,

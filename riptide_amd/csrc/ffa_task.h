@@ -126,12 +126,21 @@ __device__ __forceinline__ void fill_merge_coef(MergeCoef* mc, int M, int levels
 
 __device__ __forceinline__ int coef_slot(int M, int t, int m) { return 2 * t + (m - (M >> t)); }
 
+// Rows in the layout of merge level t of a cone (the rows that level
+// writes; level `levels` is the staged bottom).
+__device__ __forceinline__ int cone_level_rows(const Seg* seg, int t)
+{
+    const int first = seg_slot(t, 0);
+    const int n = 1 << t;
+    return seg[first + n - 1].base + seg[first + n - 1].len;
+}
+
 // Builds the cone of a tile top-down with one wavefront (no workgroup
 // barrier): level 0 is the tile itself; level t+1 holds the rows of the
-// children that level t reads.  Writes R_levels (rows at the bottom) to
-// *bottom_rows.  A level has at most 2^kMaxConeLevels slots, two per lane.
+// children that level t reads.  Writes the rows of every level t <= levels
+// to lrows[t].  A level has at most 2^kMaxConeLevels slots, two per lane.
 __device__ __forceinline__ void build_cone_wave(Seg* seg, const FfaTask& tk, const MergeCoef* mc,
-                                                int* bottom_rows, int lane)
+                                                int* lrows, int lane)
 {
     static_assert((2 << (kMaxConeLevels - 1)) <= 2 * kWave, "cone level wider than a wavefront");
     const int M = tk.node_m;
@@ -144,7 +153,6 @@ __device__ __forceinline__ void build_cone_wave(Seg* seg, const FfaTask& tk, con
         r.base = 0;
         seg[0] = r;
     }
-    int bottom = tk.hi - tk.lo;
     for (int t = 0; t < tk.levels; ++t) {
         // the slots of level t were written by other lanes of this wavefront
         __builtin_amdgcn_wave_barrier();
@@ -181,9 +189,10 @@ __device__ __forceinline__ void build_cone_wave(Seg* seg, const FfaTask& tk, con
             seg[seg_slot(t + 1, 2 * j)] = a;
             seg[seg_slot(t + 1, 2 * j + 1)] = b;
         }
-        bottom = __builtin_amdgcn_readlane(incl, kWave - 1);
     }
-    if (lane == 0) *bottom_rows = bottom;
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane <= tk.levels) lrows[lane] = cone_level_rows(seg, lane);
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
 }
@@ -254,14 +263,6 @@ __device__ __forceinline__ void cone_row_map(const Seg* seg, int t, int p, int b
 }
 
 
-// Rows in the layout of merge level t of a task (the rows that level writes).
-__device__ __forceinline__ int level_rows(bool cone, const Seg* seg, int t, int rows)
-{
-    if (!cone) return rows;
-    const int first = seg_slot(t, 0);
-    const int n = 1 << t;
-    return seg[first + n - 1].base + seg[first + n - 1].len;
-}
 
 // 32-bit LDS address of a pointer into LDS, and a load from one.
 typedef __attribute__((address_space(3))) const float lds_cfloat;
@@ -353,7 +354,7 @@ struct RowCore {
     // (four reads and three adds per element, the row tables of both levels
     // composed per row): half the barriers and table rounds, 5/6 of the LDS
     // traffic.  Row -1 of `data` is the -0.0f row, the tail of copied rows.
-    static __device__ __forceinline__ int merge(const Seg* seg, float* data, const uint32_t* tab,
+    static __device__ __forceinline__ int merge(const int* lrows, float* data, const uint32_t* tab,
                                                 int rows, int levels, bool cone, int bins,
                                                 int pitch, int last_pitch, int cap_rows, int wave,
                                                 int lane, gfloat* gdst)
@@ -364,7 +365,7 @@ struct RowCore {
             // levels t and t - 1 in one pass over LDS where two remain
             const bool two = K >= 6 && t >= 1;
             const int to = two ? t - 1 : t;  // the level this pass writes
-            const int out_rows = level_rows(cone, seg, to, rows);
+            const int out_rows = cone ? lrows[to] : rows;
             const uint32_t* tl = tab + (size_t)to * cap_rows;
             const int nr = (out_rows - wave + NW - 1) / NW;
             // this wavefront's row-table entries, all read before the first use
@@ -513,7 +514,7 @@ struct RowCore {
     // (tasks are sized so that they do not).  The row tables are built by
     // `tables` while this wavefront's row loads are in flight.
     template <class Tables, class Setup>
-    static __device__ __forceinline__ int run(const FfaTask& tk, const Seg* seg, float* data,
+    static __device__ __forceinline__ int run(const FfaTask& tk, const int* lrows, float* data,
                                               const uint32_t* tab, uint32_t grow_v, int rows,
                                               int pitch, int last_pitch, int wave, int lane,
                                               Tables&& tables, Setup&& setup)
@@ -537,7 +538,7 @@ struct RowCore {
         RT_PHASE(2);
         const int levels = __builtin_amdgcn_readfirstlane(tk.levels);
         gfloat* dst = tk.kind == 0 ? as_global(tk.dst) + ((size_t)tk.node_r0 + tk.lo) * bins : nullptr;
-        rows = merge(seg, data, tab, rows, levels, tk.cone != 0, bins, pitch, last_pitch,
+        rows = merge(lrows, data, tab, rows, levels, tk.cone != 0, bins, pitch, last_pitch,
                      __builtin_amdgcn_readfirstlane(tk.pad), wave, lane, dst);
         RT_PHASE(tk.cone ? 4 : 3);
         if (tk.kind == 0 && (levels == 0 || K < 6)) {
@@ -557,13 +558,13 @@ struct RowCore {
 // class for k > kFfaMaxExactK.  Returns -1 for a k the variant does not hold
 // (the host groups work by variant, so this does not happen).
 template <int NW, int STAGE, int V, class Tables, class Setup>
-__device__ __forceinline__ int run_rows(const FfaTask& tk, const Seg* seg, float* data,
+__device__ __forceinline__ int run_rows(const FfaTask& tk, const int* lrows, float* data,
                                         const uint32_t* tab, uint32_t grow_v, int rows,
                                         int pitch, int last_pitch, int wave, int lane,
                                         Tables&& tables, Setup&& setup)
 {
     const int k = (tk.bins + kWave - 1) / kWave;
-#define RT_CORE_ARGS tk, seg, data, tab, grow_v, rows, pitch, last_pitch, wave, lane, tables, setup
+#define RT_CORE_ARGS tk, lrows, data, tab, grow_v, rows, pitch, last_pitch, wave, lane, tables, setup
     if constexpr (V < kFfaWideVariant) {
         constexpr int KA = 2 * V, KB = 2 * V + 1;
         static_assert(KB <= kFfaMaxExactK, "variant beyond the exact column counts");
@@ -1006,7 +1007,8 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     float* data = lds + pitch;
     uint32_t* tab = reinterpret_cast<uint32_t*>(data + (size_t)cap_rows * pitch);
     Seg* seg = reinterpret_cast<Seg*>(tab + (size_t)cap_rows * kFfaMaxLevels);
-    __shared__ int s_bottom;
+    // rows of every merge level of a cone task
+    __shared__ int s_lrows[kMaxConeLevels + 1];
 
     __shared__ MergeCoef s_coef;
     const int levels = __builtin_amdgcn_readfirstlane(tk.levels);
@@ -1017,6 +1019,8 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
         if (tid == 0) atomicOr(err, 1);
         return;
     }
+    // the task's tile record: its cone and row tables come from memory
+    const int32_t* rec = tk.rec;
 
     for (int c = tid; c < bins; c += NT) lds[c] = -0.0f;
 
@@ -1025,14 +1029,19 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
 
     // ---- merge coefficients and the cone (one wavefront, one barrier):
     // before the row loads of a cone task (they need the cone), while they
-    // are in flight for a whole-node task
+    // are in flight for a whole-node task.  With a record: only the level
+    // rows, read before the staging barrier that precedes their first use.
     auto setup = [&]() {
+        if (rec) {
+            if (tk.cone && wave == 0 && lane <= levels) s_lrows[lane] = as_global(rec)[1 + lane];
+            return;
+        }
         if (wave == 0) {
             fill_merge_coef(&s_coef, M, levels, lane);
             if (tk.cone) {
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                build_cone_wave(seg, tk, &s_coef, &s_bottom, lane);
+                build_cone_wave(seg, tk, &s_coef, s_lrows, lane);
             }
         }
         __syncthreads();
@@ -1042,7 +1051,8 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
     int rows;  // rows in the current level's layout
     uint32_t grow_v = 0;
     if (tk.cone) {
-        rows = (levels > 0) ? s_bottom : (tk.hi - tk.lo);
+        rows = (levels == 0) ? (tk.hi - tk.lo)
+                             : __builtin_amdgcn_readfirstlane(rec ? as_global(rec)[0] : s_lrows[levels]);
         if (rows > cap_rows) {
             if (tid == 0) atomicOr(err, 1);
             __syncthreads();
@@ -1052,21 +1062,29 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
         // row (wave + i*NW), the row its loads are issued from
         const int p = wave + lane * NW;
         if (p < rows) {
-            const Seg g = seg[seg_find(seg, levels, p)];
-            grow_v = (uint32_t)(g.r0 + g.lo + (p - g.base));
+            if (rec) {
+                grow_v = (uint32_t)(tk.node_r0 + as_global(rec)[kRecHeader + p]);
+            } else {
+                const Seg g = seg[seg_find(seg, levels, p)];
+                grow_v = (uint32_t)(g.r0 + g.lo + (p - g.base));
+            }
         }
     } else {
         rows = M;
     }
-    // the row tables of every level, built while the row loads are in flight
+    // the row tables of every level, built (or copied from the record) while
+    // the row loads are in flight
     const int rows0 = rows;
     auto tables = [&, rows0]() {
-        if (tk.cone) {
+        if (rec) {
+            gcint* src = as_global(rec) + kRecHeader + cap_rows;
+            for (int e = tid; e < levels * cap_rows; e += NT) tab[e] = (uint32_t)src[e];
+        } else if (tk.cone) {
             // one flat loop over (level, row) so that every thread has work
             for (int e = tid; e < levels * cap_rows; e += NT) {
                 const int t = e / cap_rows;
                 const int p = e - t * cap_rows;
-                if (p < level_rows(true, seg, t, 0)) {
+                if (p < s_lrows[t]) {
                     int a, b, sh;
                     cone_row_map(seg, t, p, bins, M, &s_coef, &a, &b, &sh);
                     tab[e] = pack_row(a, b, sh);
@@ -1096,7 +1114,7 @@ __device__ __forceinline__ void run_ffa_task(const FfaTask& tk, float* lds, int*
             ext_w = wmax;
         }
     }
-    rows = run_rows<NW, STAGE, V>(tk, seg, data, tab, grow_v, rows, pitch, spitch, wave, lane,
+    rows = run_rows<NW, STAGE, V>(tk, s_lrows, data, tab, grow_v, rows, pitch, spitch, wave, lane,
                                   tables, [&]() {
                                       if (!tk.cone) setup();
                                   });
@@ -1165,9 +1183,16 @@ __device__ __forceinline__ bool tree_node(int m, int d, int i, int* r0, int* nm)
 // r0, nm) caches the rows of node `node` of pass p (nm = 0: a slot under a
 // leaf).
 __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int32_t* depth,
-                                              int& p, int& i, int& lo, int& j, int& node, int& r0,
-                                              int& nm, float* buf_a, float* buf_b, FfaTask& tk)
+                                              const int32_t* rec_first, int& p, int& i, int& lo,
+                                              int& j, int& node, int& r0, int& nm, float* buf_a,
+                                              float* buf_b, FfaTask& tk)
 {
+    // the tile record of (pass, node size class, tile), if the shape has records
+    auto record = [&](int pass, int d, int k) -> const int32_t* {
+        if (tr.rec == nullptr) return nullptr;
+        const int c = nm - (tr.rows >> d);
+        return tr.rec + (int64_t)(rec_first[2 * pass + c] + k) * rec_words(tr.cap);
+    };
     for (;;) {
         if (p >= tr.npass) return false;
         if (tr.npass == 1) {
@@ -1179,6 +1204,7 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
             tk.levels = tr.levels0;
             tk.cone = 0;
             tk.kind = 1;
+            tk.rec = tr.rec ? tr.rec + (int64_t)rec_first[0] * rec_words(tr.cap) : nullptr;
             p = 1;
             return true;
         }
@@ -1226,6 +1252,7 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
             tk.levels = h;
             tk.cone = 0;
             tk.kind = 0;
+            tk.rec = record(0, d, 0);
             ++i;
             return true;
         }
@@ -1246,6 +1273,7 @@ __device__ __forceinline__ bool next_ffa_task(const FfaTransform& tr, const int3
         tk.levels = L;
         tk.cone = 1;
         tk.kind = last ? 1 : 0;
+        tk.rec = record(p, d, lo / K);
         lo += K;
         return true;
     }
@@ -1290,10 +1318,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(WPS, WPS))) 
         tk.wmax = tr.wmax;
         tk.ext_rows = tr.ext_rows;
         int p = 0, i = 0, lo = 0, j = 0, node = -1, r0 = 0, nm = 0;
-        // the pass depths are indexed dynamically: read them from memory, so
-        // that the descriptor itself stays in registers
+        // the pass depths and record offsets are indexed dynamically: read
+        // them from memory, so that the descriptor itself stays in registers
         const int32_t* depth = trs[item].depth;
-        while (next_ffa_task(tr, depth, p, i, lo, j, node, r0, nm, buf_a, buf_b, tk)) {
+        const int32_t* rec_first = &trs[item].rec_first[0][0];
+        while (next_ffa_task(tr, depth, rec_first, p, i, lo, j, node, r0, nm, buf_a, buf_b, tk)) {
             RT_PHASE(0);
             run_ffa_task<NT, STAGE, V>(tk, lds, err);
         }

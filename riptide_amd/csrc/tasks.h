@@ -117,6 +117,32 @@ struct FfaTask {
     // values (rows at pitch ffa_snr_pitch); 0: no extension
     int32_t wmax;
     int32_t ext_rows;
+    // the task's tile record (FfaRecord layout), or null: the task builds its
+    // cone and row tables itself
+    const int32_t* rec;
+};
+
+// A tile record: what a task would otherwise compute before and while its
+// rows load, laid out in kRecWords(cap) int32 words --
+//   [0]                    rows of the bottom level (the staged rows)
+//   [1 + t], t <= levels   rows of merge level t (cone tasks)
+//   [kRecHeader + p]       node-relative source row of staged row p (cone)
+//   [kRecHeader + cap + t*cap + p]   packed row-table entry of level t, row p
+// Tiles of nodes of equal size at equal depth of one transform shape share a
+// record, and so do the equal transforms of every series of a batch.
+constexpr int kRecHeader = 16;
+constexpr int64_t rec_words(int cap) { return kRecHeader + (int64_t)cap * (1 + kFfaMaxLevels); }
+
+// Transforms of a plan shared by at least this many series of a batch get
+// tile records.
+constexpr int kRecMinSeries = 2;
+
+// One tile record to build: the task fields it depends on.
+struct FfaRecordSpec {
+    int64_t off;        // word offset of the record
+    int32_t bins, cap;
+    int32_t node_m, lo, hi, levels, cone;
+    int32_t pad;
 };
 
 // One whole FFA transform for the persistent batch kernel: the pass split
@@ -142,6 +168,11 @@ struct FfaTransform {
     int32_t wmax;           // as FfaTask::wmax / ext_rows
     int32_t ext_rows;
     int32_t k_last;         // tile rows of the last (S/N) cone pass
+    // tile records of this transform's shape (null: none): the record of
+    // pass p, node size class c (node rows = (rows >> depth[p]) + c) and
+    // tile k (lo = k * tile rows) is rec + (rec_first[p][c] + k) * rec_words(cap)
+    const int32_t* rec;
+    int32_t rec_first[kMaxPasses][2];
 };
 
 // One downsampling step of one series for the packed multi-step kernel.

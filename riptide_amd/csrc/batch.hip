@@ -21,6 +21,7 @@
 #include "ffa_plan.h"
 #include "plan.h"
 #include "series_kernels.h"
+#include "task_iter.h"
 #include "tasks.h"
 
 namespace rt {
@@ -132,6 +133,8 @@ struct Batch {
     std::vector<int> tr_series;
     std::vector<int> tr_plan_index;      // transform index within its plan
     std::vector<int64_t> tr_rec_off;     // per transform: word offset of its records, or -1
+    std::vector<int64_t> tr_task_off;    // per transform: first entry of its task list, or -1
+    std::vector<FfaTaskDesc> task_descs; // task lists of the transforms several series share
     std::vector<FfaRecordSpec> rec_specs;  // tile records shared by equal transforms
     int64_t rec_words_total = 0;
     std::vector<DsSeriesDev> host_ds;
@@ -145,6 +148,7 @@ struct Batch {
     DsStepDev* d_ds_steps = nullptr;
     int32_t* d_rec = nullptr;
     FfaRecordSpec* d_rec_specs = nullptr;
+    FfaTaskDesc* d_task_descs = nullptr;
     PrepSeriesDev* d_prep_series = nullptr;
     float* d_ds = nullptr;
     float* d_prep = nullptr;
@@ -172,7 +176,7 @@ static void free_batch(Batch* b)
     if (b->graph_stream) hipStreamDestroy(b->graph_stream);
     if (b->exec) hipGraphExecDestroy(b->exec);
     if (b->graph) hipGraphDestroy(b->graph);
-    for (void* p : {(void*)b->d_rec, (void*)b->d_rec_specs, (void*)b->d_tables, (void*)b->d_trs, (void*)b->d_ds_series,
+    for (void* p : {(void*)b->d_rec, (void*)b->d_rec_specs, (void*)b->d_task_descs, (void*)b->d_tables, (void*)b->d_trs, (void*)b->d_ds_series,
                     (void*)b->d_ds_steps, (void*)b->d_prep_series, (void*)b->d_ds,
                     (void*)b->d_prep,
                     (void*)b->d_scratch, (void*)b->d_work, (void*)b->d_queue, (void*)b->d_err})
@@ -230,6 +234,44 @@ static int64_t plan_records(Batch* b, FfaTransform* f)
         }
     }
     return off0;
+}
+
+// The task list of one transform (its records, if any, planned): the tasks
+// next_ffa_task gives, as descriptors.  Returns the first entry, or -1 if a
+// task does not fit the descriptor's fields.
+static int64_t plan_task_list(Batch* b, const FfaTransform& f, int32_t* ntasks)
+{
+    // stand-in addresses: the enumeration only compares and offsets them
+    static const float kInput[1] = {0}, kBufA[1] = {0}, kBufB[1] = {0};
+    static const int32_t kRec[1] = {0};
+    FfaTransform tr = f;
+    tr.src = kInput;
+    tr.rec = kRec;
+    float* buf_a = const_cast<float*>(kBufA);
+    float* buf_b = const_cast<float*>(kBufB);
+    const int64_t first = (int64_t)b->task_descs.size();
+    FfaTask tk = {};
+    int p = 0, i = 0, lo = 0, j = 0, node = -1, r0 = 0, nm = 0;
+    while (next_ffa_task(tr, tr.depth, &tr.rec_first[0][0], p, i, lo, j, node, r0, nm, buf_a,
+                         buf_b, tk)) {
+        FfaTaskDesc d = {};
+        d.node_r0 = tk.node_r0;
+        d.node_m = tk.node_m;
+        d.lo_rows = tk.lo | ((tk.hi - tk.lo) << 16);
+        const int src = tk.src == buf_a ? 1 : tk.src == buf_b ? 2 : 0;
+        const int dst = tk.dst == buf_b ? 1 : 0;
+        const int64_t rec = (tk.rec - kRec) / rec_words(f.cap);
+        if (rec >= (1 << 19) || tk.levels > 15 || tk.lo >= (1 << 16) || tk.hi - tk.lo >= (1 << 15)) {
+            b->task_descs.resize((size_t)first);
+            *ntasks = 0;
+            return -1;
+        }
+        d.flags = tk.levels | (tk.cone << 4) | (tk.kind << 5) | (src << 6) | (dst << 8) |
+                  (int32_t)(rec << 12);
+        b->task_descs.push_back(d);
+    }
+    *ntasks = (int32_t)((int64_t)b->task_descs.size() - first);
+    return first;
 }
 
 static int scrunch_factor(int64_t width_samples, int64_t minpts)
@@ -337,7 +379,8 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
     // get tile records, built once per run and read by every one of them
     std::vector<int> plan_series_count(b->plans.size(), 0);
     for (int64_t i = 0; i < nseries; ++i) ++plan_series_count[b->plan_of[i]];
-    std::map<std::pair<int, int>, std::pair<int64_t, FfaTransform>> rec_of;  // (plan, tr) -> records
+    // (plan, transform) -> (record offset, task list offset, transform with its record offsets)
+    std::map<std::pair<int, int>, std::tuple<int64_t, int64_t, FfaTransform>> rec_of;
 
     // groups of series; per group: packed downsampling descriptors and the
     // transform queue sorted by decreasing cost
@@ -356,7 +399,7 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
         const int tr_base = (int)b->host_trs.size();
         std::vector<FfaTransform> local;
         std::vector<int> local_series, local_pidx;
-        std::vector<int64_t> local_rec;
+        std::vector<int64_t> local_rec, local_task;
         for (int s = g.s0; s < g.s1; ++s) {
             const SeriesPlan& p = b->plans[b->plan_of[s]];
             b->ds_off[s] = ds_run;
@@ -425,23 +468,34 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                         f.cap, depths[depths.size() - 2] - depths[depths.size() - 1], f.ext_rows);
                     max_fold = std::max<int64_t>(max_fold, (int64_t)t.rows * t.bins);
                 }
-                int64_t rec_off = -1;
-                if (plan_series_count[b->plan_of[s]] >= kRecMinSeries) {
+                int64_t rec_off = -1, task_off = -1;
+                if (plan_series_count[b->plan_of[s]] < kRecMinSeries) {
+                    task_off = plan_task_list(b, f, &f.ntasks);
+                } else {
                     const auto key = std::make_pair(b->plan_of[s], (int)it);
                     auto rt = rec_of.find(key);
                     if (rt == rec_of.end()) {
                         FfaTransform g = f;
                         const int64_t off = plan_records(b, &g);
-                        rt = rec_of.emplace(key, std::make_pair(off, g)).first;
+                        const int64_t toff = plan_task_list(b, g, &g.ntasks);
+                        rt = rec_of.emplace(key, std::make_tuple(off, toff, g)).first;
                     }
-                    rec_off = rt->second.first;
-                    std::memcpy(f.rec_first, rt->second.second.rec_first, sizeof(f.rec_first));
+                    rec_off = std::get<0>(rt->second);
+                    task_off = std::get<1>(rt->second);
+                    const FfaTransform& g = std::get<2>(rt->second);
+                    std::memcpy(f.rec_first, g.rec_first, sizeof(f.rec_first));
+                    f.ntasks = g.ntasks;
+                }
+                if (task_off < 0) {
+                    free_batch(b);
+                    return bfail(RT_EINVAL, "transform too tall for the batch task list");
                 }
                 order.emplace_back(transform_cost(t), (int)local.size());
                 local.push_back(f);
                 local_series.push_back(s);
                 local_pidx.push_back((int)it);
                 local_rec.push_back(rec_off);
+                local_task.push_back(task_off);
             }
         }
         // by kernel variant and workgroup shape, then largest first
@@ -470,6 +524,7 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
             b->tr_series.push_back(local_series[o.second]);
             b->tr_plan_index.push_back(local_pidx[o.second]);
             b->tr_rec_off.push_back(local_rec[o.second]);
+            b->tr_task_off.push_back(local_task[o.second]);
         }
         g.tr0 = tr_base;
         g.ntr = (int)order.size();
@@ -538,8 +593,16 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
         BT_HIP(hipMemcpy(b->d_rec_specs, b->rec_specs.data(),
                          b->rec_specs.size() * sizeof(FfaRecordSpec), hipMemcpyHostToDevice));
     }
-    for (size_t k = 0; k < b->host_trs.size(); ++k)
+    if (!b->task_descs.empty()) {
+        BT_HIP(hipMalloc(&b->d_task_descs, b->task_descs.size() * sizeof(FfaTaskDesc)));
+        BT_HIP(hipMemcpy(b->d_task_descs, b->task_descs.data(),
+                         b->task_descs.size() * sizeof(FfaTaskDesc), hipMemcpyHostToDevice));
+    }
+    for (size_t k = 0; k < b->host_trs.size(); ++k) {
         b->host_trs[k].rec = b->tr_rec_off[k] >= 0 ? b->d_rec + b->tr_rec_off[k] : nullptr;
+        b->host_trs[k].tasks =
+            b->tr_task_off[k] >= 0 ? b->d_task_descs + b->tr_task_off[k] : nullptr;
+    }
     *out = b;
     return RT_OK;
 }

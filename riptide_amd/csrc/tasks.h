@@ -145,6 +145,19 @@ struct FfaRecordSpec {
     int32_t pad;
 };
 
+// A task of a transform in the task list the host builds for the batch
+// kernel (transforms several series share use one list): what varies
+// between its tasks.
+//   flags: levels (bits 0-3) | cone << 4 | kind << 5 | source (bits 6-7: 0
+//   the folded input, 1 buf_a, 2 buf_b) | destination << 8 (0 buf_a, 1
+//   buf_b) | record index << 12
+struct FfaTaskDesc {
+    int32_t node_r0;
+    int32_t node_m;
+    int32_t lo_rows;    // lo | (hi - lo) << 16
+    int32_t flags;
+};
+
 // One whole FFA transform for the persistent batch kernel: the pass split
 // (depth[0] = leaf depth of pass 0, ..., depth[npass-1] = 0) is precomputed by
 // the host planner; tiles and nodes are enumerated on the device.
@@ -173,6 +186,9 @@ struct FfaTransform {
     // tile k (lo = k * tile rows) is rec + (rec_first[p][c] + k) * rec_words(cap)
     const int32_t* rec;
     int32_t rec_first[kMaxPasses][2];
+    // the transform's tasks in the order next_ffa_task gives them
+    const FfaTaskDesc* tasks;
+    int32_t ntasks;
 };
 
 // One downsampling step of one series for the packed multi-step kernel.

@@ -100,3 +100,51 @@ def test_batch_tiling_classes_match_oracle(oracle):
             assert np.array_equal(Pb, P) and np.array_equal(Fb, F)
             np.testing.assert_allclose(_lib.to_numpy(outs[i]), S, rtol=1e-5, atol=1e-4)
     b.close()
+
+
+def test_batch_shared_plans_use_tile_records(oracle):
+    """Series of equal length and bins range share one plan: their transforms
+    take the tile records (cones and row tables built once per run) and a
+    shared task list.  Every series matches the oracle, and a series searched
+    in a batch of its own (no record: the tasks build their tables
+    themselves, same tiling) gives bit-identical S/N."""
+    import torch
+    from riptide_amd import _lib
+    from riptide_amd.batch import BatchSearch
+
+    rng = np.random.default_rng(99)
+    tsamp = 128e-6
+    sizes = [2 ** 18, 2 ** 18, 2 ** 17 + 321, 2 ** 18, 2 ** 17 + 321, 200_000]
+    bmin = np.array([480, 480, 240, 480, 240, 240])
+    bmax = np.where(bmin == 240, 260, 520)
+    data = [_series(rng, n, tsamp) for n in sizes]
+    kw = dict(rmed_width=1.0)
+    b = BatchSearch(sizes, tsamp, bmin, bmax, 0.08, 4.0, group_size=2, **kw)
+    assert b.stats()["tile_records"] > 0
+    xs = [_lib.to_device(x) for x in data]
+    outs = b.allocate_outputs()
+    b.bind(xs, outs)
+    for use_graph in (False, True, True):
+        for o in outs:
+            o.fill_(np.nan)
+        b.run(use_graph=use_graph)
+        b.check()
+        torch.cuda.synchronize()
+        for i, x in enumerate(data):
+            _, w, P, F, S = oracle.ffa_search(x, tsamp, period_min=0.08, period_max=4.0,
+                                              bins_min=int(bmin[i]), bins_max=int(bmax[i]), **kw)
+            Pb, Fb = b.grid(i)
+            assert np.array_equal(Pb, P) and np.array_equal(Fb, F)
+            np.testing.assert_allclose(_lib.to_numpy(outs[i]), S, rtol=1e-5, atol=1e-4)
+    got = [_lib.to_numpy(o) for o in outs]
+    b.close()
+    for i in (0, 2):
+        one = BatchSearch([sizes[i]], tsamp, bmin[i:i + 1], bmax[i:i + 1], 0.08, 4.0, **kw)
+        assert one.stats()["tile_records"] == 0
+        o1 = one.allocate_outputs()
+        one.bind([xs[i]], o1)
+        one.run(use_graph=True)
+        one.check()
+        torch.cuda.synchronize()
+        assert np.array_equal(_lib.to_numpy(o1[0]), got[i])
+        one.close()

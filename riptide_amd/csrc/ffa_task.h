@@ -753,13 +753,38 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         if (lane < 2 * nwl) hb_ld = as_global(tk.hb)[lane];
     }
     const int g4 = bins >> 2;
+    // a thread's item (chunk c, row p) and its chunk's bounds: worked out
+    // once for every pass where each thread has at most one item (integer
+    // divisions are long instruction sequences)
+    const bool one = nitems <= NT;
+    const int c0 = tid / nrows;
+    const int p0 = tid - c0 * nrows;
+    const int j00 = 4 * ((c0 * g4) / C);
+    const int j01 = c0 == C - 1 ? bins : 4 * (((c0 + 1) * g4) / C);
+    auto item = [&](int it, int& c, int& p) {
+        if (one) {
+            c = c0;
+            p = p0;
+        } else {
+            c = it / nrows;
+            p = it - c * nrows;
+        }
+    };
+    auto bounds = [&](int c, int& j, int& je) {
+        if (one) {
+            j = j00;
+            je = j01;
+        } else {
+            j = 4 * ((c * g4) / C);
+            je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
+        }
+    };
     // (1) chunk totals, accumulated in float64 as the reference does
     for (int it = tid; it < nitems; it += NT) {
-        const int c = it / nrows;
-        const int p = it - c * nrows;
+        int c, p, j, je;
+        item(it, c, p);
+        bounds(c, j, je);
         const float* row = data + (size_t)p * pitch;
-        int j = 4 * ((c * g4) / C);
-        const int je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
         double acc = 0.0;
         for (; j + 4 <= je; j += 4) {
             float x[4];
@@ -783,13 +808,12 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
     // only read them, so no barrier is needed before the rows are
     // overwritten.
     for (int it = tid; it < nitems; it += NT) {
-        const int c = it / nrows;
-        const int p = it - c * nrows;
+        int c, p, j, je;
+        item(it, c, p);
+        bounds(c, j, je);
         double acc = 0.0;
         for (int q = 0; q < c; ++q) acc += *tot_at(q, p);
         float* row = data + (size_t)p * pitch;
-        int j = 4 * ((c * g4) / C);
-        const int je = c == C - 1 ? bins : 4 * (((c + 1) * g4) / C);
         for (; j + 4 <= je; j += 4) {
             float x[4];
 #pragma unroll
@@ -851,12 +875,14 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
         // extension, else [0, bins - wmax)
         const int P = ext_w ? bins : max(0, bins - wmax);
         const int q4 = P >> 2;
+        const int i00 = 4 * ((c0 * q4) / C);
+        const int i01 = c0 == C - 1 ? P : 4 * (((c0 + 1) * q4) / C);
         for (int it = tid; it < nitems; it += NT) {
-            const int c = it / nrows;
-            const int p = it - c * nrows;
+            int c, p;
+            item(it, c, p);
             const float* cps = data + (size_t)p * pitch;
-            const int i0 = 4 * ((c * q4) / C);
-            const int i1 = c == C - 1 ? P : 4 * (((c + 1) * q4) / C);
+            const int i0 = one ? i00 : 4 * ((c * q4) / C);
+            const int i1 = one ? i01 : c == C - 1 ? P : 4 * (((c + 1) * q4) / C);
             if (i0 >= i1) continue;
             float m[kSnrMaxWidths];
 #pragma unroll

@@ -476,7 +476,11 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                     auto rt = rec_of.find(key);
                     if (rt == rec_of.end()) {
                         FfaTransform g = f;
-                        const int64_t off = plan_records(b, &g);
+                        // records up to a memory budget; past it the tasks
+                        // of further shapes build their tables themselves
+                        const int64_t off = b->rec_words_total * 4 < kRecMaxBytes
+                                                ? plan_records(b, &g)
+                                                : -1;
                         const int64_t toff = plan_task_list(b, g, &g.ntasks);
                         rt = rec_of.emplace(key, std::make_tuple(off, toff, g)).first;
                     }

@@ -136,6 +136,9 @@ constexpr int64_t rec_words(int cap) { return kRecHeader + (int64_t)cap * (1 + k
 // Transforms of a plan shared by at least this many series of a batch get
 // tile records.
 constexpr int kRecMinSeries = 2;
+// Device memory the tile records of one batch may take (shapes planned
+// after it is reached get none).
+constexpr int64_t kRecMaxBytes = (int64_t)4 << 30;
 
 // One tile record to build: the task fields it depends on.
 struct FfaRecordSpec {

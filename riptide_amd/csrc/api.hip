@@ -887,7 +887,7 @@ int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const ui
 namespace rt {
 
 constexpr size_t kNoAllocScratchBytes = (size_t)256 << 20;
-// part of it kept for the task list (about 280k tasks)
+// part of it kept for the task list (about 240k tasks)
 constexpr size_t kNoAllocTaskBytes = (size_t)24 << 20;
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];

@@ -148,3 +148,46 @@ def test_batch_shared_plans_use_tile_records(oracle):
         torch.cuda.synchronize()
         assert np.array_equal(_lib.to_numpy(o1[0]), got[i])
         one.close()
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2])
+def test_batch_shared_plans_every_shape(oracle, shape):
+    """Shared plans (tile records, shared task lists) under every workgroup
+    shape, with series that share a plan but not their trial widths (the
+    widest width fixes the S/N tiling of the first series' task list; the
+    others fall back where their rows do not fit), against the oracle."""
+    import torch
+    from riptide_amd import _lib, tune
+    from riptide_amd.batch import BatchSearch
+    from riptide_amd.ffautils import generate_width_trials
+
+    rng = np.random.default_rng(1000 + shape)
+    tsamp = 128e-6
+    sizes = [200_000] * 4 + [150_000] * 2
+    bmin = np.array([480, 480, 480, 480, 240, 240])
+    bmax = np.where(bmin == 240, 260, 520)
+    widths = [generate_width_trials(int(b)) for b in bmin]
+    widths[1] = np.array([1, 2, 5, 11, 30], np.uint64)   # same plan as 0, other widths
+    widths[3] = np.array([3, 7, 40, 80], np.uint64)
+    data = [_series(rng, n, tsamp) for n in sizes]
+    try:
+        b = BatchSearch(sizes, tsamp, bmin, bmax, 0.08, 3.0, widths=widths, group_size=3,
+                        shape=shape, rmed_width=1.0)
+        assert b.stats()["tile_records"] > 0
+        xs = [_lib.to_device(x) for x in data]
+        outs = b.allocate_outputs()
+        b.bind(xs, outs)
+        b.run(use_graph=True)
+        b.check()
+        torch.cuda.synchronize()
+        from oracle import oracle as O
+        for i, x in enumerate(data):
+            xn = O.normalise(O.deredden(np.asarray(x, np.float32), tsamp, 1.0, 101))
+            P, F, S = O.oracle().periodogram(xn, tsamp, np.asarray(widths[i], np.uint64), 0.08,
+                                             3.0, int(bmin[i]), int(bmax[i]))
+            Pb, Fb = b.grid(i)
+            assert np.array_equal(Pb, P) and np.array_equal(Fb, F)
+            np.testing.assert_allclose(_lib.to_numpy(outs[i]), S, rtol=1e-5, atol=1e-4)
+        b.close()
+    finally:
+        tune.set_shape(None)

@@ -78,7 +78,10 @@ int rt_snr2(const float* d_in, int64_t rows, int64_t cols, const uint64_t* h_wid
 int rt_downsample(const float* d_in, int64_t size, double f, float* d_out, void* stream);
 
 /* libcpp.periodogram (python_bindings.cpp:180-209 -> periodogram.hpp:117-201):
- * d_periods / d_foldbins / d_snrs hold length, length and length x nw values. */
+ * d_periods / d_foldbins / d_snrs hold length, length and length x nw values.
+ * Runs as a batch of one series (rt_batch_*, with the batch tiling table in
+ * effect); the batch of the last search is kept for the next call with the
+ * same search.  Waits for the stream. */
 int rt_periodogram(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
                    int64_t nw, double period_min, double period_max, int64_t bins_min,
                    int64_t bins_max, double* d_periods, uint32_t* d_foldbins, float* d_snrs,

@@ -804,6 +804,92 @@ static int64_t pgram_tasks(const PgramSetup& su, const float* d_in, int s0, int 
     return used;
 }
 
+std::vector<int32_t> batch_tiling_flat();
+
+// A one-series search runs as a batch of one series: the persistent batch
+// kernel, with the batch's task lists and tile records built once.  The batch
+// of the last search is kept, with its period and fold-bins grids in device
+// memory, and rebound to each call's input and output while the search
+// (size, sampling time, widths, period and bins ranges, FFA shape and batch
+// tiling table) stays the same.
+struct OneSeriesBatch {
+    void* handle = nullptr;
+    int64_t size = 0, bmin = 0, bmax = 0;
+    double tsamp = 0.0, pmin = 0.0, pmax = 0.0;
+    int shape = -1;
+    std::vector<uint64_t> widths;
+    std::vector<int32_t> tiling;
+    int64_t length = 0;
+    double* d_periods = nullptr;
+    uint32_t* d_foldbins = nullptr;
+};
+static OneSeriesBatch g_one;
+
+static void drop_one_series_batch()
+{
+    OneSeriesBatch& c = g_one;
+    if (c.handle) rt_batch_destroy(c.handle);
+    if (c.d_periods) hipFree(c.d_periods);
+    if (c.d_foldbins) hipFree(c.d_foldbins);
+    c = OneSeriesBatch();
+}
+
+// The batch of this search in *out (made on a miss), or null when the batch
+// path does not take the search (then the caller runs it pass by pass).
+// Returns an error code for a search that is not valid, as the pass path.
+static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths, int64_t nw,
+                            double pmin, double pmax, int64_t bmin, int64_t bmax,
+                            OneSeriesBatch** out)
+{
+    *out = nullptr;
+    const int shape = current_ffa_shape();
+    std::vector<uint64_t> w(h_widths, h_widths + std::max<int64_t>(nw, 0));
+    std::vector<int32_t> tiling = batch_tiling_flat();
+    OneSeriesBatch& c = g_one;
+    if (c.handle && c.size == size && c.tsamp == tsamp && c.pmin == pmin && c.pmax == pmax &&
+        c.bmin == bmin && c.bmax == bmax && c.shape == shape && c.widths == w && c.tiling == tiling) {
+        *out = &c;
+        return RT_OK;
+    }
+    SeriesPlan plan;
+    const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &plan);
+    if (!e.empty()) return fail(RT_EINVAL, e);
+    const int st = check_widths(h_widths, nw, bmin);
+    if (st) return st;
+    if (plan.length == 0 || nw == 0) return RT_OK;
+    drop_one_series_batch();
+    void* h = nullptr;
+    const int64_t count = nw;
+    if (rt_batch_create(1, &size, &tsamp, h_widths, &count, pmin, pmax, &bmin, &bmax, 0, 0.0, 0,
+                        0, 1, &h) != RT_OK)
+        return RT_OK;
+    c.handle = h;
+    std::vector<double> periods((size_t)plan.length);
+    std::vector<uint32_t> foldbins((size_t)plan.length);
+    plan_periods(plan, periods.data(), foldbins.data());
+    if (hipMalloc(&c.d_periods, periods.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c.d_foldbins, foldbins.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMemcpy(c.d_periods, periods.data(), periods.size() * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c.d_foldbins, foldbins.data(), foldbins.size() * sizeof(uint32_t),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        drop_one_series_batch();
+        return RT_OK;
+    }
+    c.size = size;
+    c.tsamp = tsamp;
+    c.pmin = pmin;
+    c.pmax = pmax;
+    c.bmin = bmin;
+    c.bmax = bmax;
+    c.shape = shape;
+    c.widths = std::move(w);
+    c.tiling = std::move(tiling);
+    c.length = plan.length;
+    *out = &c;
+    return RT_OK;
+}
+
 }  // namespace rt
 
 extern "C" {
@@ -814,6 +900,25 @@ int rt_periodogram(const float* d_in, int64_t size, double tsamp, const uint64_t
 {
     std::lock_guard<std::mutex> lk(g_pool_mu);
     hipStream_t s = S(stream);
+    {
+        // the batch of one series, where the batch path takes the search
+        OneSeriesBatch* ob = nullptr;
+        const int st = one_series_batch(size, tsamp, h_widths, nw, pmin, pmax, bmin, bmax, &ob);
+        if (st) return st;
+        if (ob) {
+            const int64_t L = ob->length;
+            RT_HIP(hipMemcpyAsync(d_periods, ob->d_periods, L * sizeof(double),
+                                  hipMemcpyDeviceToDevice, s));
+            RT_HIP(hipMemcpyAsync(d_foldbins, ob->d_foldbins, L * sizeof(uint32_t),
+                                  hipMemcpyDeviceToDevice, s));
+            const float* in = d_in;
+            float* out = d_snrs;
+            int bst = rt_batch_bind(ob->handle, &in, &out);
+            if (!bst) bst = rt_batch_run(ob->handle, 0, s);
+            const int cst = rt_batch_check(ob->handle, s);
+            return bst ? bst : cst;
+        }
+    }
     PgramSetup su;
     int st = pgram_setup(d_in, size, tsamp, h_widths, nw, pmin, pmax, bmin, bmax, d_periods,
                          d_foldbins, s, &su);

@@ -106,6 +106,16 @@ struct Group {
 
 }  // namespace
 
+// The tiling table in effect, 7 values per class (what rt_batch_set_tiling
+// took): batches created under equal tables tile alike.
+std::vector<int32_t> batch_tiling_flat()
+{
+    std::vector<int32_t> out;
+    for (const TileClass& c : g_tile_classes)
+        out.insert(out.end(), {c.rows_lo, c.rows_hi, c.bins_lo, c.bins_hi, c.shape, c.cap, c.lmax});
+    return out;
+}
+
 struct Batch {
     int device = 0;
     int64_t nseries = 0;

@@ -175,15 +175,21 @@ def periodogram_device(d_data, size, tsamp, widths, period_min, period_max, bins
     foldbins = _lib.empty((L,), "uint32")
     snrs = _lib.empty((L, w.size))
     name = "rt_periodogram_plain" if plain else "rt_periodogram"
-    # FFA workgroup shape chosen by the autotuner for this bins range, if any
+    # FFA workgroup shape chosen by the autotuner for this bins range, if any;
+    # rt_periodogram runs the search as a batch of one series, tiled as the
+    # tuned (rows, bins) classes say (none while a shape is forced)
     shape = tune.shape_for("periodogram", int(bins_min))
     if shape is not None:
         _lib.call("rt_set_ffa_shape", shape)
+    if not plain:
+        tune.set_batch_tiling(tune.batch_tiling())
     try:
         _lib.call(name, _lib.ptr(d_data), int(size), float(tsamp), _lib.host_ptr(w), w.size,
                   float(period_min), float(period_max), int(bins_min), int(bins_max),
                   _lib.ptr(periods), _lib.ptr(foldbins), _lib.ptr(snrs), _lib.stream())
     finally:
+        if not plain:
+            tune.set_batch_tiling([])
         if shape is not None:
             tune.restore()
     return periods, foldbins, snrs

@@ -188,16 +188,27 @@ SHAPE_CASES = [
 ]
 
 
+def _periodogram(x, tsamp, w, pmin, pmax, bmin, bmax, plain):
+    """The drop-in periodogram (a batch of one series through the batch
+    kernel) or, with plain, the per-step schedule of the task kernel."""
+    from riptide_amd import _lib, libcpp
+    d = _lib.to_device(np.ascontiguousarray(x, np.float32))
+    P, F, S = libcpp.periodogram_device(d, x.size, tsamp, np.asarray(w, np.uint64), pmin, pmax,
+                                        bmin, bmax, plain=plain)
+    return _lib.to_numpy(P), _lib.to_numpy(F).view(np.uint32), _lib.to_numpy(S)
+
+
+@pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("shape", [0, 1, 2])
 @pytest.mark.parametrize("case", SHAPE_CASES)
-def test_periodogram_parity_shapes(oracle, case, shape):
-    from riptide_amd import libcpp, tune
+def test_periodogram_parity_shapes(oracle, case, shape, plain):
+    from riptide_amd import tune
     N, tsamp, pmin, pmax, bmin, bmax, widths = case
     x = oracle.normalise(_pulsar(N, tsamp, 0.37 * (pmin + pmax), 0.05, 15.0, seed=bmin + shape))
     w = oracle.generate_width_trials(bmin) if widths is None else np.asarray(widths)
     tune.set_shape(shape)
     try:
-        P, F, S = libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+        P, F, S = _periodogram(x, tsamp, w, pmin, pmax, bmin, bmax, plain)
     finally:
         tune.set_shape(None)
     Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
@@ -222,10 +233,11 @@ DEFAULT_WIDTH_CASES = [
 ]
 
 
+@pytest.mark.parametrize("plain", [False, True])
 @pytest.mark.parametrize("shape", [0, 1, 2])
 @pytest.mark.parametrize("bmin,N", DEFAULT_WIDTH_CASES)
-def test_periodogram_default_width_counts(oracle, bmin, N, shape):
-    from riptide_amd import libcpp, tune
+def test_periodogram_default_width_counts(oracle, bmin, N, shape, plain):
+    from riptide_amd import tune
     tsamp = 1e-3
     pmin = bmin * tsamp * 1.05
     pmax = 1.6 * pmin
@@ -235,7 +247,7 @@ def test_periodogram_default_width_counts(oracle, bmin, N, shape):
     x = oracle.normalise(_pulsar(N, tsamp, 1.3 * pmin, 0.04, 15.0, seed=bmin + 10 * shape))
     tune.set_shape(shape)
     try:
-        P, F, S = libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+        P, F, S = _periodogram(x, tsamp, w, pmin, pmax, bmin, bmax, plain)
     finally:
         tune.set_shape(None)
     Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)

@@ -810,10 +810,11 @@ std::vector<int32_t> batch_tiling_flat();
 // kernel, with the batch's task lists and tile records built once.  The batch
 // of the last search is kept, with its period and fold-bins grids in device
 // memory, and rebound to each call's input and output while the search
-// (size, sampling time, widths, period and bins ranges, FFA shape and batch
-// tiling table) stays the same.
+// (device, size, sampling time, widths, period and bins ranges, FFA shape and
+// batch tiling table) stays the same.
 struct OneSeriesBatch {
     void* handle = nullptr;
+    int device = -1;
     int64_t size = 0, bmin = 0, bmax = 0;
     double tsamp = 0.0, pmin = 0.0, pmax = 0.0;
     int shape = -1;
@@ -842,11 +843,13 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
                             OneSeriesBatch** out)
 {
     *out = nullptr;
+    int device = 0;
+    RT_HIP(hipGetDevice(&device));
     const int shape = current_ffa_shape();
     std::vector<uint64_t> w(h_widths, h_widths + std::max<int64_t>(nw, 0));
     std::vector<int32_t> tiling = batch_tiling_flat();
     OneSeriesBatch& c = g_one;
-    if (c.handle && c.size == size && c.tsamp == tsamp && c.pmin == pmin && c.pmax == pmax &&
+    if (c.handle && c.device == device && c.size == size && c.tsamp == tsamp && c.pmin == pmin && c.pmax == pmax &&
         c.bmin == bmin && c.bmax == bmax && c.shape == shape && c.widths == w && c.tiling == tiling) {
         *out = &c;
         return RT_OK;
@@ -857,6 +860,12 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
     const int st = check_widths(h_widths, nw, bmin);
     if (st) return st;
     if (plan.length == 0 || nw == 0) return RT_OK;
+    // (the batch of another device is freed on that device)
+    if (c.handle && c.device != device) {
+        RT_HIP(hipSetDevice(c.device));
+        drop_one_series_batch();
+        RT_HIP(hipSetDevice(device));
+    }
     drop_one_series_batch();
     void* h = nullptr;
     const int64_t count = nw;
@@ -864,6 +873,7 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
                         0, 1, &h) != RT_OK)
         return RT_OK;
     c.handle = h;
+    c.device = device;
     std::vector<double> periods((size_t)plan.length);
     std::vector<uint32_t> foldbins((size_t)plan.length);
     plan_periods(plan, periods.data(), foldbins.data());

@@ -849,8 +849,9 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
     std::vector<uint64_t> w(h_widths, h_widths + std::max<int64_t>(nw, 0));
     std::vector<int32_t> tiling = batch_tiling_flat();
     OneSeriesBatch& c = g_one;
-    if (c.handle && c.device == device && c.size == size && c.tsamp == tsamp && c.pmin == pmin && c.pmax == pmax &&
-        c.bmin == bmin && c.bmax == bmax && c.shape == shape && c.widths == w && c.tiling == tiling) {
+    if (c.handle && c.device == device && c.size == size && c.tsamp == tsamp && c.pmin == pmin &&
+        c.pmax == pmax && c.bmin == bmin && c.bmax == bmax && c.shape == shape && c.widths == w &&
+        c.tiling == tiling) {
         *out = &c;
         return RT_OK;
     }
@@ -870,8 +871,10 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
     void* h = nullptr;
     const int64_t count = nw;
     if (rt_batch_create(1, &size, &tsamp, h_widths, &count, pmin, pmax, &bmin, &bmax, 0, 0.0, 0,
-                        0, 1, &h) != RT_OK)
+                        0, 1, &h) != RT_OK) {
+        (void)hipGetLastError();  // a failed allocation must not fail the pass path's launches
         return RT_OK;
+    }
     c.handle = h;
     c.device = device;
     std::vector<double> periods((size_t)plan.length);
@@ -884,6 +887,7 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
         hipMemcpy(c.d_foldbins, foldbins.data(), foldbins.size() * sizeof(uint32_t),
                   hipMemcpyHostToDevice) != hipSuccess) {
         drop_one_series_batch();
+        (void)hipGetLastError();
         return RT_OK;
     }
     c.size = size;

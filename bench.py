@@ -615,6 +615,34 @@ def main():
         log(f"plain: {plain_ms[0]:.2f} ms per 240-260 trial, {plain_ms[1]:.2f} ms per 480-520 "
             f"trial -> {plain_tps:.1f} trials/s at the batch's class mix")
 
+        # the drop-in one-series path (libcpp.periodogram, what ffa_search
+        # calls): a kept batch of one series per call; same series, same
+        # dereddening and normalisation per series, one untimed call per
+        # class first (it builds the kept batch)
+        dropin_ms = {}
+        for cls in (0, 1):
+            idx = [i for i in range(args.trials) if coin[i] == cls][:per_class]
+
+            def dropin_run(ids):
+                for i in ids:
+                    b0, b1 = int(bmin[i]), int(bmax[i])
+                    _lib.call("rt_deredden", _lib.ptr(xs[i]), args.nsamp, ws, 101, _lib.ptr(tmp),
+                              _lib.stream())
+                    _lib.call("rt_normalise", _lib.ptr(tmp), args.nsamp, _lib.ptr(tmp2),
+                              _lib.stream())
+                    libcpp.periodogram_device(tmp2, args.nsamp, tsamp, widths[b0], pmin, pmax,
+                                              b0, b1)
+
+            dropin_run(idx[:1])
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            dropin_run(idx)
+            torch.cuda.synchronize()
+            dropin_ms[cls] = (time.perf_counter() - t0) / max(1, len(idx)) * 1e3
+        dropin_tps = 1e3 / ((1 - frac480) * dropin_ms[0] + frac480 * dropin_ms[1])
+        log(f"drop-in: {dropin_ms[0]:.2f} ms per 240-260 trial, {dropin_ms[1]:.2f} ms per "
+            f"480-520 trial -> {dropin_tps:.1f} trials/s at the batch's class mix")
+
     # heterogeneous batch: 512 series, lengths log-uniform 2^21..2^23 rounded
     # to a multiple of 1024, same noise model and bins coin
     c3 = None
@@ -748,6 +776,13 @@ def main():
                                "graph; timed per bins class under every FFA workgroup shape, "
                                "the fastest kept, and weighted by the batch's class mix")},
         "batched_over_plain": round(trials_per_s / plain_tps, 3),
+        "dropin": {"value": round(dropin_tps, 2), "unit": "DM trials/s", "trials": nplain,
+                   "ms_per_trial_by_class": {"240-260": round(dropin_ms[0], 3),
+                                             "480-520": round(dropin_ms[1], 3)},
+                   "schedule": ("the drop-in one-series path (libcpp.periodogram, as ffa_search "
+                                "calls it): a kept batch of one series per call, eager launches; "
+                                "same series, dereddening and normalisation as the plain line")},
+        "batched_over_dropin": round(trials_per_s / dropin_tps, 3),
         "phases_ms": {k: round(v, 3) if isinstance(v, float) else v for k, v in phases.items()},
         "parity": checks["parity"] if checks else None,
         "snr_vs_fp64": checks["snr_vs_fp64"] if checks else None,

@@ -279,3 +279,20 @@ def test_ffa_search_results_stay_on_device(oracle):
     _, pg2 = rt.ffa_search(tsdr, period_min=0.5, period_max=2.0, bins_min=240, bins_max=260,
                            deredden=False, already_normalised=True)
     np.testing.assert_array_equal(pg2.snrs, pg.snrs)
+
+
+def test_dropin_periodogram_kept_batch(oracle):
+    """The drop-in periodogram keeps the batch of its last search: a second
+    series with the same search (the kept batch, rebound), another search in
+    between (a new batch) and the first search again all equal the oracle."""
+    N, tsamp = 2 ** 17, 256e-6
+    a = (0.5, 2.0, 240, 260)
+    b = (0.4, 1.6, 480, 520)
+    x1 = oracle.normalise(_pulsar(N, tsamp, 0.83, 0.03, 15.0, seed=21))
+    x2 = oracle.normalise(_pulsar(N, tsamp, 1.27, 0.03, 15.0, seed=22))
+    for x, (pmin, pmax, bmin, bmax) in ((x1, a), (x2, a), (x1, b), (x2, a)):
+        w = oracle.generate_width_trials(bmin)
+        P, F, S = _periodogram(x, tsamp, w, pmin, pmax, bmin, bmax, False)
+        Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+        assert np.array_equal(P, Po) and np.array_equal(F, Fo)
+        np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)

@@ -14,8 +14,10 @@ or a batch), so it is measured:
 times, on synthetic white noise of the given shape,
  (a) the single-series periodogram under every shape, per bins class;
  (b) a batch of K series under every (shape, lmax), per (rows class, bins
-     class): the period range is cut so that the batch's transforms have
-     rows in the class (rows ~ nsamp * tsamp / period);
+     class), then the best of those under every tile cap of CAP_CHOICES
+     (rows per workgroup, below what fits): the period range is cut so that
+     the batch's transforms have rows in the class (rows ~ nsamp * tsamp /
+     period);
 and writes the winners to a JSON file (default: ``tuning.json`` next to this
 module, or ``$RIPTIDE_AMD_TUNING``).  ``shape_for("periodogram", bins_min)``
 and ``batch_tiling()`` read it; the periodogram entry points and
@@ -36,6 +38,9 @@ DEFAULT_CLASSES = ((240, 260), (480, 520))
 ROWS_CLASSES = ((1, 128), (128, 1024), (1024, 1 << 30))
 # merge levels per cone pass tried by the batch tuner (the planner allows 7)
 LMAX_CHOICES = (2, 3, 4, 5)
+# tile caps (rows per workgroup; 0: as many as fit) tried with the best
+# (shape, lmax) of a class; a cap above what fits is clamped to it
+CAP_CHOICES = (0, 96, 80, 72, 64, 56, 48)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 _cache = None
@@ -134,15 +139,15 @@ def _time_periodogram(shape, nsamp, tsamp, bins, pmin, pmax, reps=5):
     return float(np.median(ts))
 
 
-def _time_batch(shape, lmax, nsamp, tsamp, bins, pmin, pmax, trials):
+def _time_batch(shape, lmax, nsamp, tsamp, bins, pmin, pmax, trials, cap=0):
     """Least of two FFA phase times of a batch of `trials` series, every
-    transform tiled with (shape, lmax)."""
+    transform tiled with (shape, lmax, cap)."""
     import torch
     from .batch import BatchSearch
     xs = [torch.randn(nsamp, device=_lib.device()) for _ in range(trials)]
     b = BatchSearch([nsamp] * trials, tsamp, bins[0], bins[1], pmin, pmax, deredden=False,
                     already_normalised=True, group_size=trials,
-                    tiling=[[0, 1 << 30, 1, 1 << 30, shape, 0, lmax]])
+                    tiling=[[0, 1 << 30, 1, 1 << 30, shape, cap, lmax]])
     outs = b.allocate_outputs()
     b.bind(xs, outs)
     b.run_timed()
@@ -196,16 +201,27 @@ def tune(nsamp=2 ** 22, tsamp=64e-6, period_min=0.1, period_max=10.0, trials=16,
             ms = {}
             for shape in range(NUM_SHAPES):
                 for lmax in LMAX_CHOICES:
-                    ms[(shape, lmax)] = _time_batch(shape, lmax, nsamp, tsamp, bins, pr[0],
-                                                    pr[1], trials)
+                    ms[(shape, lmax, 0)] = _time_batch(shape, lmax, nsamp, tsamp, bins, pr[0],
+                                                       pr[1], trials)
+            # the tile cap, with the best (shape, lmax)
+            s0, l0, _ = min(ms, key=ms.get)
+            for cap in CAP_CHOICES:
+                if (s0, l0, cap) not in ms:
+                    ms[(s0, l0, cap)] = _time_batch(s0, l0, nsamp, tsamp, bins, pr[0], pr[1],
+                                                    trials, cap)
             best = min(ms, key=ms.get)
+            # a cap below what fits only where it is more than 1% faster than
+            # the full tile (closer timings are within the run-to-run spread)
+            if best[2] and ms[best] > 0.99 * ms[(s0, l0, 0)]:
+                best = (s0, l0, 0)
             table["batch_tiles"].append({
                 "rows": [rows[0], rows[1]], "bins": [bins[0], bins[1]], "shape": best[0],
-                "cap": 0, "lmax": best[1], "periods": [round(pr[0], 6), round(pr[1], 6)],
-                "ffa_ms": {f"shape{k[0]}_lmax{k[1]}": round(v, 3) for k, v in ms.items()}})
+                "cap": best[2], "lmax": best[1], "periods": [round(pr[0], 6), round(pr[1], 6)],
+                "ffa_ms": {(f"shape{k[0]}_lmax{k[1]}" + (f"_cap{k[2]}" if k[2] else "")):
+                           round(v, 3) for k, v in ms.items()}})
             log(f"bins {key}, rows {rows[0]}-{rows[1]} (P {pr[0]:.3g}-{pr[1]:.3g} s): best "
-                f"shape {best[0]}, lmax {best[1]}: {ms[best]:.2f} ms; shape 0, lmax 5: "
-                f"{ms[(0, 5)]:.2f} ms")
+                f"shape {best[0]}, lmax {best[1]}, cap {best[2]}: {ms[best]:.2f} ms; shape 0, "
+                f"lmax 5: {ms[(0, 5, 0)]:.2f} ms")
     set_shape(None)
     path = out or tuning_path()
     with open(path, "w") as f:

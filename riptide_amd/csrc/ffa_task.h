@@ -841,8 +841,8 @@ __device__ __forceinline__ void snr_rows(const FfaTask& tk, float* data, int pit
     if (nw >= 1 && nw <= kSnrMaxWidths && nrows * nw <= scratch_floats) {
         for (int e = tid; e < nrows * nw; e += NT) mx[e] = ordered_int(-INFINITY);
         // widths and (h + b, b) pairs were stored to LDS after the first
-        // prefix pass; one LDS read per lane (lane j: width j), then the widths from the
-        // lanes, so that the reads are not serialised
+        // prefix pass; one LDS read per lane (lane j: width j), then the
+        // widths from the lanes, so that the reads are not serialised
         const int w_lane = s_w[min(lane, nw - 1)];
         int w[kSnrMaxWidths];
 #pragma unroll

@@ -391,6 +391,10 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
     for (int64_t i = 0; i < nseries; ++i) ++plan_series_count[b->plan_of[i]];
     // (plan, transform) -> (record offset, task list offset, transform with its record offsets)
     std::map<std::pair<int, int>, std::tuple<int64_t, int64_t, FfaTransform>> rec_of;
+    // task lists of transforms without records, shared by every transform of
+    // the same tiling (rows, rows evaluated, tile rows, pass depths), whatever
+    // its series or bins: (key) -> (task list offset, tasks)
+    std::map<std::vector<int32_t>, std::pair<int64_t, int32_t>> list_of;
 
     // groups of series; per group: packed downsampling descriptors and the
     // transform queue sorted by decreasing cost
@@ -480,7 +484,17 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                 }
                 int64_t rec_off = -1, task_off = -1;
                 if (plan_series_count[b->plan_of[s]] < kRecMinSeries) {
-                    task_off = plan_task_list(b, f, &f.ntasks);
+                    std::vector<int32_t> key = {f.rows, f.rows_eval, f.cap, f.npass, f.levels0,
+                                                f.k_last};
+                    for (int q = 0; q < f.npass && q < kMaxPasses; ++q) key.push_back(f.depth[q]);
+                    auto lt = list_of.find(key);
+                    if (lt == list_of.end()) {
+                        int32_t n = 0;
+                        const int64_t off = plan_task_list(b, f, &n);
+                        lt = list_of.emplace(std::move(key), std::make_pair(off, n)).first;
+                    }
+                    task_off = lt->second.first;
+                    f.ntasks = lt->second.second;
                 } else {
                     const auto key = std::make_pair(b->plan_of[s], (int)it);
                     auto rt = rec_of.find(key);

@@ -43,6 +43,13 @@ int rt_version(void);
  * only speed differs (riptide_amd/tune.py picks the faster). */
 int rt_set_ffa_shape(int shape);
 
+/* Releases the device memory the library keeps between calls on the current
+ * device: the scratch pool of the pass-by-pass schedules and the kept batch
+ * of the last one-series search (rt_periodogram).  No counterpart in the
+ * reference, which allocates inside every call; for callers that need that
+ * memory back between searches.  Waits for the device. */
+int rt_release_scratch(void);
+
 /* ---- Host-side search plan (no device work) ------------------------------ */
 
 /* periodogram_length (periodogram.hpp:63-109; python_bindings.cpp:196):

@@ -13,6 +13,7 @@ from .libffa import ffa1, ffa2, ffafreq, ffaprd, generate_signal, downsample, bo
 from .ffautils import generate_width_trials
 from .peak_detection import Peak, find_peaks
 from .serialization import save_json, load_json
+from ._lib import release_scratch
 
 __version__ = "0.1.0"
 
@@ -35,4 +36,5 @@ __all__ = [
     "find_peaks",
     "save_json",
     "load_json",
+    "release_scratch",
 ]

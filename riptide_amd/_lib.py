@@ -24,6 +24,7 @@ _c_ptr = ctypes.c_void_p
 _SIGNATURES = {
     "rt_version": [],
     "rt_set_ffa_shape": [ctypes.c_int],
+    "rt_release_scratch": [],
     "rt_periodogram_length": [_c_i64, _c_dbl, _c_dbl, _c_dbl, _c_i64, _c_i64, _c_ptr],
     "rt_periodogram_grid": [_c_i64, _c_dbl, _c_dbl, _c_dbl, _c_i64, _c_i64, _c_ptr, _c_ptr],
     "rt_downsample_index": [_c_i64, _c_dbl, _c_i64, _c_ptr, _c_ptr, _c_ptr, _c_ptr],
@@ -122,6 +123,13 @@ def call(name, *args):
     if st == RT_EINVAL:
         raise ValueError(msg)
     raise RuntimeError(msg)
+
+
+def release_scratch():
+    """Gives back the device memory the library keeps between calls on the
+    current device (its scratch pool and the kept batch of the last
+    one-series periodogram); later calls allocate again as they need."""
+    call("rt_release_scratch")
 
 
 # ---------------------------------------------------------------------------

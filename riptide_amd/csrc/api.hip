@@ -953,6 +953,29 @@ int rt_periodogram(const float* d_in, int64_t size, double tsamp, const uint64_t
     return run_passes(passes, s, true, su.shape);
 }
 
+int rt_release_scratch(void)
+{
+    std::lock_guard<std::mutex> lk(g_pool_mu);
+    int dev = 0;
+    RT_HIP(hipGetDevice(&dev));
+    RT_HIP(hipDeviceSynchronize());
+    if (g_one.handle && g_one.device == dev) drop_one_series_batch();
+    auto it = g_pool.find(dev);
+    if (it != g_pool.end()) {
+        hipError_t err = hipSuccess;
+        for (PoolBuf& b : it->second) {
+            if (b.p) {
+                const hipError_t e = hipFree(b.p);
+                if (e != hipSuccess) err = e;
+            }
+            b = PoolBuf();
+        }
+        g_pool.erase(it);
+        RT_HIP(err);
+    }
+    return RT_OK;
+}
+
 int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
                          int64_t nw, double pmin, double pmax, int64_t bmin, int64_t bmax,
                          double* d_periods, uint32_t* d_foldbins, float* d_snrs, void* stream)

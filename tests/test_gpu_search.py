@@ -296,3 +296,24 @@ def test_dropin_periodogram_kept_batch(oracle):
         Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
         assert np.array_equal(P, Po) and np.array_equal(F, Fo)
         np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)
+
+
+def test_release_scratch(oracle):
+    """release_scratch gives back the kept batch of the drop-in periodogram
+    (device memory rises by about its size) and later searches still equal
+    the oracle."""
+    import torch
+    import riptide_amd as rt
+    N, tsamp, pmin, pmax, bmin, bmax = 2 ** 18, 256e-6, 0.5, 2.0, 480, 520
+    x = oracle.normalise(_pulsar(N, tsamp, 0.91, 0.03, 15.0, seed=31))
+    w = oracle.generate_width_trials(bmin)
+    Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    for _ in range(2):
+        P, F, S = _periodogram(x, tsamp, w, pmin, pmax, bmin, bmax, False)
+        assert np.array_equal(P, Po) and np.array_equal(F, Fo)
+        np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)
+        torch.cuda.synchronize()
+        free0 = torch.cuda.mem_get_info()[0]
+        rt.release_scratch()
+        free1 = torch.cuda.mem_get_info()[0]
+        assert free1 > free0

@@ -44,8 +44,8 @@ int rt_version(void);
 int rt_set_ffa_shape(int shape);
 
 /* Releases the device memory the library keeps between calls on the current
- * device: the scratch pool of the pass-by-pass schedules and the kept batch
- * of the last one-series search (rt_periodogram).  No counterpart in the
+ * device: the scratch pool of the pass-by-pass schedules and the kept batches
+ * of the last one-series searches (rt_periodogram).  No counterpart in the
  * reference, which allocates inside every call; for callers that need that
  * memory back between searches.  Waits for the device. */
 int rt_release_scratch(void);
@@ -87,8 +87,8 @@ int rt_downsample(const float* d_in, int64_t size, double f, float* d_out, void*
 /* libcpp.periodogram (python_bindings.cpp:180-209 -> periodogram.hpp:117-201):
  * d_periods / d_foldbins / d_snrs hold length, length and length x nw values.
  * Runs as a batch of one series (rt_batch_*, with the batch tiling table in
- * effect); the batch of the last search is kept for the next call with the
- * same search.  Waits for the stream. */
+ * effect); the batches of the last four searches are kept for later calls
+ * with one of those searches.  Waits for the stream. */
 int rt_periodogram(const float* d_in, int64_t size, double tsamp, const uint64_t* h_widths,
                    int64_t nw, double period_min, double period_max, int64_t bins_min,
                    int64_t bins_max, double* d_periods, uint32_t* d_foldbins, float* d_snrs,

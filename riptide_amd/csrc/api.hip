@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <list>
 #include <map>
 #include <mutex>
 #include <string>
@@ -807,11 +808,12 @@ static int64_t pgram_tasks(const PgramSetup& su, const float* d_in, int s0, int 
 std::vector<int32_t> batch_tiling_flat();
 
 // A one-series search runs as a batch of one series: the persistent batch
-// kernel, with the batch's task lists and tile records built once.  The batch
-// of the last search is kept, with its period and fold-bins grids in device
-// memory, and rebound to each call's input and output while the search
-// (device, size, sampling time, widths, period and bins ranges, FFA shape and
-// batch tiling table) stays the same.
+// kernel, with the batch's task lists and tile records built once.  The
+// batches of the last few searches are kept, with their period and fold-bins
+// grids in device memory, and rebound to each call's input and output while
+// a search (device, size, sampling time, widths, period and bins ranges, FFA
+// shape and batch tiling table) repeats -- as it does when every DM trial is
+// searched over the same few period ranges.
 struct OneSeriesBatch {
     void* handle = nullptr;
     int device = -1;
@@ -824,15 +826,69 @@ struct OneSeriesBatch {
     double* d_periods = nullptr;
     uint32_t* d_foldbins = nullptr;
 };
-static OneSeriesBatch g_one;
+// searches kept (most recently used first)
+constexpr size_t kKeptSearches = 4;
+static std::list<OneSeriesBatch> g_kept;
 
-static void drop_one_series_batch()
+// Frees the device memory of a kept batch (on its own device).
+static void drop_one_series_batch(OneSeriesBatch& c)
 {
-    OneSeriesBatch& c = g_one;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    const bool other = c.device >= 0 && c.device != cur;
+    if (other) (void)hipSetDevice(c.device);
     if (c.handle) rt_batch_destroy(c.handle);
-    if (c.d_periods) hipFree(c.d_periods);
-    if (c.d_foldbins) hipFree(c.d_foldbins);
+    if (c.d_periods) (void)hipFree(c.d_periods);
+    if (c.d_foldbins) (void)hipFree(c.d_foldbins);
+    if (other) (void)hipSetDevice(cur);
     c = OneSeriesBatch();
+}
+
+// Drops the kept batches of device `dev` (all devices: -1).
+static void drop_kept_batches(int dev)
+{
+    for (auto it = g_kept.begin(); it != g_kept.end();) {
+        if (dev < 0 || it->device == dev) {
+            drop_one_series_batch(*it);
+            it = g_kept.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+
+// Makes the batch and grids of a search into c: RT_OK, RT_EINVAL when the
+// batch path does not take the search, RT_EDEVICE when device memory runs
+// out (c left empty but for those).
+static int make_one_series_batch(OneSeriesBatch& c, const SeriesPlan& plan, int64_t size,
+                                  double tsamp, const uint64_t* h_widths, int64_t nw,
+                                  double pmin, double pmax, int64_t bmin, int64_t bmax)
+{
+    void* h = nullptr;
+    const int64_t count = nw;
+    const int st = rt_batch_create(1, &size, &tsamp, h_widths, &count, pmin, pmax, &bmin, &bmax,
+                                   0, 0.0, 0, 0, 1, &h);
+    if (st != RT_OK) {
+        (void)hipGetLastError();  // a failed allocation must not fail later launches
+        return st == RT_EINVAL ? RT_EINVAL : RT_EDEVICE;
+    }
+    (void)hipGetDevice(&c.device);
+    c.handle = h;
+    std::vector<double> periods((size_t)plan.length);
+    std::vector<uint32_t> foldbins((size_t)plan.length);
+    plan_periods(plan, periods.data(), foldbins.data());
+    if (hipMalloc(&c.d_periods, periods.size() * sizeof(double)) != hipSuccess ||
+        hipMalloc(&c.d_foldbins, foldbins.size() * sizeof(uint32_t)) != hipSuccess ||
+        hipMemcpy(c.d_periods, periods.data(), periods.size() * sizeof(double),
+                  hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c.d_foldbins, foldbins.data(), foldbins.size() * sizeof(uint32_t),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+        drop_one_series_batch(c);
+        (void)hipGetLastError();
+        return RT_EDEVICE;
+    }
+    c.length = plan.length;
+    return RT_OK;
 }
 
 // The batch of this search in *out (made on a miss), or null when the batch
@@ -848,12 +904,15 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
     const int shape = current_ffa_shape();
     std::vector<uint64_t> w(h_widths, h_widths + std::max<int64_t>(nw, 0));
     std::vector<int32_t> tiling = batch_tiling_flat();
-    OneSeriesBatch& c = g_one;
-    if (c.handle && c.device == device && c.size == size && c.tsamp == tsamp && c.pmin == pmin &&
-        c.pmax == pmax && c.bmin == bmin && c.bmax == bmax && c.shape == shape && c.widths == w &&
-        c.tiling == tiling) {
-        *out = &c;
-        return RT_OK;
+    for (auto it = g_kept.begin(); it != g_kept.end(); ++it) {
+        const OneSeriesBatch& c = *it;
+        if (c.device == device && c.size == size && c.tsamp == tsamp && c.pmin == pmin &&
+            c.pmax == pmax && c.bmin == bmin && c.bmax == bmax && c.shape == shape &&
+            c.widths == w && c.tiling == tiling) {
+            g_kept.splice(g_kept.begin(), g_kept, it);
+            *out = &g_kept.front();
+            return RT_OK;
+        }
     }
     SeriesPlan plan;
     const std::string e = plan_series(size, tsamp, pmin, pmax, bmin, bmax, &plan);
@@ -861,35 +920,18 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
     const int st = check_widths(h_widths, nw, bmin);
     if (st) return st;
     if (plan.length == 0 || nw == 0) return RT_OK;
-    // (the batch of another device is freed on that device)
-    if (c.handle && c.device != device) {
-        RT_HIP(hipSetDevice(c.device));
-        drop_one_series_batch();
-        RT_HIP(hipSetDevice(device));
+    if (g_kept.size() >= kKeptSearches) {
+        drop_one_series_batch(g_kept.back());
+        g_kept.pop_back();
     }
-    drop_one_series_batch();
-    void* h = nullptr;
-    const int64_t count = nw;
-    if (rt_batch_create(1, &size, &tsamp, h_widths, &count, pmin, pmax, &bmin, &bmax, 0, 0.0, 0,
-                        0, 1, &h) != RT_OK) {
-        (void)hipGetLastError();  // a failed allocation must not fail the pass path's launches
-        return RT_OK;
+    OneSeriesBatch c;
+    int made = make_one_series_batch(c, plan, size, tsamp, h_widths, nw, pmin, pmax, bmin, bmax);
+    if (made == RT_EDEVICE && !g_kept.empty()) {
+        // out of device memory: without the kept batches of this device, once more
+        drop_kept_batches(device);
+        made = make_one_series_batch(c, plan, size, tsamp, h_widths, nw, pmin, pmax, bmin, bmax);
     }
-    c.handle = h;
-    c.device = device;
-    std::vector<double> periods((size_t)plan.length);
-    std::vector<uint32_t> foldbins((size_t)plan.length);
-    plan_periods(plan, periods.data(), foldbins.data());
-    if (hipMalloc(&c.d_periods, periods.size() * sizeof(double)) != hipSuccess ||
-        hipMalloc(&c.d_foldbins, foldbins.size() * sizeof(uint32_t)) != hipSuccess ||
-        hipMemcpy(c.d_periods, periods.data(), periods.size() * sizeof(double),
-                  hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(c.d_foldbins, foldbins.data(), foldbins.size() * sizeof(uint32_t),
-                  hipMemcpyHostToDevice) != hipSuccess) {
-        drop_one_series_batch();
-        (void)hipGetLastError();
-        return RT_OK;
-    }
+    if (made != RT_OK) return RT_OK;  // pass by pass
     c.size = size;
     c.tsamp = tsamp;
     c.pmin = pmin;
@@ -899,8 +941,8 @@ static int one_series_batch(int64_t size, double tsamp, const uint64_t* h_widths
     c.shape = shape;
     c.widths = std::move(w);
     c.tiling = std::move(tiling);
-    c.length = plan.length;
-    *out = &c;
+    g_kept.push_front(std::move(c));
+    *out = &g_kept.front();
     return RT_OK;
 }
 
@@ -959,7 +1001,7 @@ int rt_release_scratch(void)
     int dev = 0;
     RT_HIP(hipGetDevice(&dev));
     RT_HIP(hipDeviceSynchronize());
-    if (g_one.handle && g_one.device == dev) drop_one_series_batch();
+    drop_kept_batches(dev);
     auto it = g_pool.find(dev);
     if (it != g_pool.end()) {
         hipError_t err = hipSuccess;

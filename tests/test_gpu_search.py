@@ -282,15 +282,22 @@ def test_ffa_search_results_stay_on_device(oracle):
 
 
 def test_dropin_periodogram_kept_batch(oracle):
-    """The drop-in periodogram keeps the batch of its last search: a second
-    series with the same search (the kept batch, rebound), another search in
-    between (a new batch) and the first search again all equal the oracle."""
+    """The drop-in periodogram keeps the batches of its last four searches:
+    a second series with the same search (a kept batch, rebound), searches
+    that alternate (each kept), more searches than are kept (the least
+    recently used dropped) and an evicted search again (a new batch) all
+    equal the oracle."""
     N, tsamp = 2 ** 17, 256e-6
     a = (0.5, 2.0, 240, 260)
     b = (0.4, 1.6, 480, 520)
+    c = (0.5, 1.9, 240, 260)
+    d = (0.6, 2.0, 250, 270)
+    e = (0.5, 1.8, 480, 520)
     x1 = oracle.normalise(_pulsar(N, tsamp, 0.83, 0.03, 15.0, seed=21))
     x2 = oracle.normalise(_pulsar(N, tsamp, 1.27, 0.03, 15.0, seed=22))
-    for x, (pmin, pmax, bmin, bmax) in ((x1, a), (x2, a), (x1, b), (x2, a)):
+    calls = ((x1, a), (x2, a), (x1, b), (x2, a), (x1, b), (x2, c), (x1, d), (x2, e), (x1, a),
+             (x2, e), (x1, b))
+    for x, (pmin, pmax, bmin, bmax) in calls:
         w = oracle.generate_width_trials(bmin)
         P, F, S = _periodogram(x, tsamp, w, pmin, pmax, bmin, bmax, False)
         Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
@@ -299,7 +306,7 @@ def test_dropin_periodogram_kept_batch(oracle):
 
 
 def test_release_scratch(oracle):
-    """release_scratch gives back the kept batch of the drop-in periodogram
+    """release_scratch gives back the kept batches of the drop-in periodogram
     (device memory rises by about its size) and later searches still equal
     the oracle."""
     import torch

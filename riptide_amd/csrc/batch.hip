@@ -102,6 +102,14 @@ struct Group {
     int total_chunks = 0;
     int halo = 2;
     int nds_series = 0;            // series with at least one packed step
+    // steps whose factor exceeds the packed kernel's limit (the longest
+    // periods at few bins): one direct downsampling launch each
+    struct DirectStep {
+        int series;
+        double f;
+        int64_t n, out_offset;
+    };
+    std::vector<DirectStep> direct;
 };
 
 }  // namespace
@@ -427,8 +435,8 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
             for (const Step& st : p.steps) {
                 if (st.ds_offset < 0 || st.n == 0) continue;
                 if (st.f > kDsPackedMaxF) {
-                    free_batch(b);
-                    return bfail(RT_EINVAL, "downsampling factor too large for the batch path");
+                    g.direct.push_back({s, st.f, st.n, st.ds_offset});
+                    continue;
                 }
                 DsStepDev ds;
                 split_double(st.f, &ds.mf, &ds.ef);
@@ -738,6 +746,12 @@ static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
         if (g.total_chunks > 0)
             BT_HIP(launch_downsample_packed(b->d_ds_series + g.s0, g.s1 - g.s0, b->d_ds_steps,
                                             g.total_chunks, kDsChunk, g.halo, s));
+        for (const Group::DirectStep& d : g.direct) {
+            const float* x = (b->deredden || b->normalise) ? b->d_prep + b->prep_off[d.series]
+                                                            : b->inputs[d.series];
+            BT_HIP(launch_downsample_direct(x, b->N[d.series], d.f, d.n,
+                                            b->d_ds + b->ds_off[d.series] + d.out_offset, s));
+        }
         if (!g.runs.empty()) {
             // queue counters of the group's runs are consecutive
             hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(64), 0, s,

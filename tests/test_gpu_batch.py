@@ -47,6 +47,42 @@ def test_batch_matches_oracle(oracle):
     b.close()
 
 
+def test_batch_large_downsampling_factors(oracle):
+    """Periods long against bins x tsamp: the steps with downsampling factors
+    above 4000 (here 6-12 s at 32-36 bins and 64 us: factors 2930-5860) take
+    the direct downsampling kernel, the others the packed one; every series
+    equals the oracle, eager and from the graph."""
+    import torch
+    from riptide_amd import _lib
+    from riptide_amd.batch import BatchSearch
+
+    rng = np.random.default_rng(4000)
+    tsamp, pmin, pmax = 64e-6, 6.0, 12.0
+    sizes = [2 ** 21, 2 ** 21 + 4096, 3 * 2 ** 20]
+    bmin = np.array([32, 40, 32])
+    bmax = np.array([36, 44, 36])
+    data = [_series(rng, n, tsamp) for n in sizes]
+    b = BatchSearch(sizes, tsamp, bmin, bmax, pmin, pmax, rmed_width=20.0, group_size=2)
+    xs = [_lib.to_device(x) for x in data]
+    outs = b.allocate_outputs()
+    b.bind(xs, outs)
+    want = [oracle.ffa_search(x, tsamp, period_min=pmin, period_max=pmax, bins_min=int(bmin[i]),
+                              bins_max=int(bmax[i]), rmed_width=20.0)
+            for i, x in enumerate(data)]
+    for use_graph in (False, True, True):
+        for o in outs:
+            o.fill_(np.nan)
+        b.run(use_graph=use_graph)
+        b.check()
+        torch.cuda.synchronize()
+        for i, (_, w, P, F, S) in enumerate(want):
+            Pb, Fb = b.grid(i)
+            assert np.array_equal(Pb, P) and np.array_equal(Fb, F)
+            assert (P / (F * tsamp)).max() > 4000  # a direct step is present
+            np.testing.assert_allclose(_lib.to_numpy(outs[i]), S, rtol=1e-5, atol=1e-4)
+    b.close()
+
+
 def test_ffa_search_batch_api(oracle):
     import riptide_amd as rt
     from riptide_amd.batch import ffa_search_batch

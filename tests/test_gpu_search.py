@@ -29,6 +29,7 @@ PGRAM_CASES = [
     (200_000, 1e-3, 0.8, 1.2, 240, 260),
     (200_000, 1e-3, 0.8, 1.2, 800, 1200),   # period_min = bins_min * tsamp: first step f == 1
     (2 ** 20, 64e-6, 0.1, 1.0, 480, 520),   # multi-pass transforms (up to ~1300 rows)
+    (2 ** 21, 64e-6, 6.0, 12.0, 32, 36),    # downsampling factors 2930-5860 (direct above 4000)
 ]
 
 
@@ -47,9 +48,11 @@ def test_periodogram_parity(oracle, case):
     np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)
 
 
-def test_plain_schedule_matches(oracle):
+@pytest.mark.parametrize("case", [(2 ** 18, 128e-6, 0.2, 2.0, 240, 260),
+                                  (2 ** 21, 64e-6, 6.0, 12.0, 32, 36)])
+def test_plain_schedule_matches(oracle, case):
     from riptide_amd import _lib, libcpp
-    N, tsamp, pmin, pmax, bmin, bmax = 2 ** 18, 128e-6, 0.2, 2.0, 240, 260
+    N, tsamp, pmin, pmax, bmin, bmax = case
     x = oracle.normalise(_pulsar(N, tsamp, 0.77, 0.02, 20.0, seed=5))
     w = oracle.generate_width_trials(bmin)
     d = _lib.to_device(x)

@@ -726,29 +726,36 @@ static int pgram_setup(const float* d_in, int64_t size, double tsamp, const uint
         }
     }
     if (!steps.empty()) {
-        if (steps.size() > (size_t)kDsMaxSteps)
-            return fail(RT_EINVAL, "too many downsampling steps");
+        // one launch per kDsMaxSteps steps (the packed kernel's LDS tables),
+        // all with the halo of the largest factor, as the batch path stages
+        // its chunks (the float-float prefix sums are cut into per-thread
+        // segments of the staged length, so the halo sets their rounding)
         const int chunk = kDsChunk;
         const int halo = (int)std::ceil(fmax) + 2;
         const int nchunks = (int)((size + chunk - 1) / chunk);
-        DsSeriesDev sd;
-        sd.x = d_in;
-        sd.out = out->d_ds;
-        sd.N = size;
-        sd.step0 = 0;
-        sd.nsteps = (int32_t)steps.size();
-        sd.chunk_offset = 0;
-        sd.pad = 0;
+        const size_t nl = (steps.size() + kDsMaxSteps - 1) / kDsMaxSteps;
+        std::vector<uint8_t> blob(nl * sizeof(DsSeriesDev) + steps.size() * sizeof(DsStepDev));
+        for (size_t l = 0; l < nl; ++l) {
+            DsSeriesDev sd;
+            sd.x = d_in;
+            sd.out = out->d_ds;
+            sd.N = size;
+            sd.step0 = (int32_t)(l * kDsMaxSteps);
+            sd.nsteps = (int32_t)std::min<size_t>(kDsMaxSteps, steps.size() - l * kDsMaxSteps);
+            sd.chunk_offset = 0;
+            sd.pad = 0;
+            std::memcpy(blob.data() + l * sizeof(DsSeriesDev), &sd, sizeof(sd));
+        }
+        std::memcpy(blob.data() + nl * sizeof(DsSeriesDev), steps.data(),
+                    steps.size() * sizeof(DsStepDev));
         // the descriptors ride in the task slot's neighbour: a misc upload
-        std::vector<uint8_t> blob(sizeof(DsSeriesDev) + steps.size() * sizeof(DsStepDev));
-        std::memcpy(blob.data(), &sd, sizeof(sd));
-        std::memcpy(blob.data() + sizeof(sd), steps.data(), steps.size() * sizeof(DsStepDev));
         uint8_t* d_blob = nullptr;
         st = upload(kSlotMisc, blob, &d_blob, s, arena);
         if (st) return st;
-        RT_HIP(launch_downsample_packed((const DsSeriesDev*)d_blob, 1,
-                                        (const DsStepDev*)(d_blob + sizeof(DsSeriesDev)), nchunks,
-                                        chunk, halo, s));
+        const DsStepDev* d_steps = (const DsStepDev*)(d_blob + nl * sizeof(DsSeriesDev));
+        for (size_t l = 0; l < nl; ++l)
+            RT_HIP(launch_downsample_packed((const DsSeriesDev*)d_blob + l, 1, d_steps, nchunks,
+                                            chunk, halo, s));
     }
     // the host staging vectors above die with this frame
     RT_HIP(hipStreamSynchronize(s));

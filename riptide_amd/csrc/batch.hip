@@ -101,6 +101,14 @@ struct Group {
     int ds0 = 0;                   // DsSeriesDev entries [ds0, ds0 + s1 - s0)
     int total_chunks = 0;
     int halo = 2;
+    // series with more packed steps than one launch holds (kDsMaxSteps):
+    // their steps [k * kDsMaxSteps, (k + 1) * kDsMaxSteps) in launch k >= 1,
+    // DsSeriesDev entries [ds0, ds0 + count) of those series; every launch
+    // of the group stages its chunks with the same halo
+    struct DsLaunch {
+        int ds0, count, total_chunks;
+    };
+    std::vector<DsLaunch> more_ds;
     int nds_series = 0;            // series with at least one packed step
     // steps whose factor exceeds the packed kernel's limit (the longest
     // periods at few bins): one direct downsampling launch each
@@ -156,6 +164,7 @@ struct Batch {
     std::vector<FfaRecordSpec> rec_specs;  // tile records shared by equal transforms
     int64_t rec_words_total = 0;
     std::vector<DsSeriesDev> host_ds;
+    std::vector<int> ds_series;            // per host_ds entry: its series
     std::vector<DsStepDev> host_steps;
     std::vector<PrepSeriesDev> host_prep;  // per series, pointers filled at bind
     std::vector<int64_t> work_off;         // per series, offset in the group work buffer
@@ -422,6 +431,7 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
         std::vector<FfaTransform> local;
         std::vector<int> local_series, local_pidx;
         std::vector<int64_t> local_rec, local_task;
+        std::vector<int> packed_steps;  // per series of the group
         for (int s = g.s0; s < g.s1; ++s) {
             const SeriesPlan& p = b->plans[b->plan_of[s]];
             b->ds_off[s] = ds_run;
@@ -447,13 +457,11 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
                 fmax = std::max(fmax, st.f);
                 ++nst;
             }
-            if (nst > kDsMaxSteps) {
-                free_batch(b);
-                return bfail(RT_EINVAL, "too many downsampling steps");
-            }
-            d.nsteps = nst;
+            d.nsteps = std::min(nst, kDsMaxSteps);
+            packed_steps.push_back(nst);
             g.total_chunks += (int)((p.size + kDsChunk - 1) / kDsChunk);
             b->host_ds.push_back(d);
+            b->ds_series.push_back(s);
             ds_run += p.ds_total;
             const int wg = b->wgroup_of[s];
             for (size_t it = 0; it < p.tr.size(); ++it) {
@@ -565,6 +573,23 @@ int rt_batch_create(int64_t nseries, const int64_t* sizes, const double* tsamps,
         g.tr0 = tr_base;
         g.ntr = (int)order.size();
         g.halo = (int)std::ceil(fmax) + 2;
+        for (int k = 1;; ++k) {
+            Group::DsLaunch l = {(int)b->host_ds.size(), 0, 0};
+            for (int i = 0; i < g.s1 - g.s0; ++i) {
+                DsSeriesDev d = b->host_ds[(size_t)g.ds0 + i];
+                const int total = packed_steps[(size_t)i];
+                if (total <= k * kDsMaxSteps) continue;
+                d.step0 += k * kDsMaxSteps;
+                d.nsteps = std::min(kDsMaxSteps, total - k * kDsMaxSteps);
+                d.chunk_offset = l.total_chunks;
+                l.total_chunks += (int)((d.N + kDsChunk - 1) / kDsChunk);
+                b->host_ds.push_back(d);
+                b->ds_series.push_back(g.s0 + i);
+                ++l.count;
+            }
+            if (!l.count) break;
+            g.more_ds.push_back(l);
+        }
         max_ds = std::max(max_ds, ds_run);
         max_prep = std::max(max_prep, prep_run);
         b->groups.push_back(g);
@@ -666,8 +691,7 @@ int rt_batch_bind(void* handle, const float* const* h_inputs, float* const* h_sn
     b->outputs.assign(h_snrs, h_snrs + b->nseries);
     const bool prep = b->deredden || b->normalise;
     for (size_t k = 0; k < b->host_ds.size(); ++k) {
-        // host_ds is in series order
-        const int s = (int)k;
+        const int s = b->ds_series[k];
         b->host_ds[k].x = prep ? b->d_prep + b->prep_off[s] : b->inputs[s];
         b->host_ds[k].out = b->d_ds + b->ds_off[s];
     }
@@ -744,8 +768,11 @@ static int enqueue_batch(Batch* b, hipStream_t s, PhaseTimer* pt = nullptr)
         }
         if (pt) BT_HIP(hipEventRecord(pt->ev[4 * gi + 1], s));
         if (g.total_chunks > 0)
-            BT_HIP(launch_downsample_packed(b->d_ds_series + g.s0, g.s1 - g.s0, b->d_ds_steps,
+            BT_HIP(launch_downsample_packed(b->d_ds_series + g.ds0, g.s1 - g.s0, b->d_ds_steps,
                                             g.total_chunks, kDsChunk, g.halo, s));
+        for (const Group::DsLaunch& l : g.more_ds)
+            BT_HIP(launch_downsample_packed(b->d_ds_series + l.ds0, l.count, b->d_ds_steps,
+                                            l.total_chunks, kDsChunk, g.halo, s));
         for (const Group::DirectStep& d : g.direct) {
             const float* x = (b->deredden || b->normalise) ? b->d_prep + b->prep_off[d.series]
                                                             : b->inputs[d.series];

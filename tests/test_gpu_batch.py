@@ -83,6 +83,41 @@ def test_batch_large_downsampling_factors(oracle):
     b.close()
 
 
+def test_batch_many_downsampling_steps(oracle):
+    """More downsampling steps than one packed launch holds (256): bins
+    240-241 over P 0.1-10 s take 555 steps, in three launches; the series
+    of a group with fewer steps (bins 240-260, 57 steps) take one. Every
+    series equals the oracle, eager and from the graph."""
+    import torch
+    from riptide_amd import _lib
+    from riptide_amd.batch import BatchSearch
+
+    rng = np.random.default_rng(555)
+    tsamp, pmin, pmax = 64e-6, 0.1, 10.0
+    sizes = [2 ** 20, 2 ** 20 + 8192, 2 ** 20]
+    bmin = np.array([240, 240, 240])
+    bmax = np.array([241, 260, 241])
+    data = [_series(rng, n, tsamp) for n in sizes]
+    b = BatchSearch(sizes, tsamp, bmin, bmax, pmin, pmax, rmed_width=20.0, group_size=3)
+    xs = [_lib.to_device(x) for x in data]
+    outs = b.allocate_outputs()
+    b.bind(xs, outs)
+    want = [oracle.ffa_search(x, tsamp, period_min=pmin, period_max=pmax, bins_min=int(bmin[i]),
+                              bins_max=int(bmax[i]), rmed_width=20.0)
+            for i, x in enumerate(data)]
+    for use_graph in (False, True, True):
+        for o in outs:
+            o.fill_(np.nan)
+        b.run(use_graph=use_graph)
+        b.check()
+        torch.cuda.synchronize()
+        for i, (_, w, P, F, S) in enumerate(want):
+            Pb, Fb = b.grid(i)
+            assert np.array_equal(Pb, P) and np.array_equal(Fb, F)
+            np.testing.assert_allclose(_lib.to_numpy(outs[i]), S, rtol=1e-5, atol=1e-4)
+    b.close()
+
+
 def test_ffa_search_batch_api(oracle):
     import riptide_amd as rt
     from riptide_amd.batch import ffa_search_batch

@@ -30,6 +30,7 @@ PGRAM_CASES = [
     (200_000, 1e-3, 0.8, 1.2, 800, 1200),   # period_min = bins_min * tsamp: first step f == 1
     (2 ** 20, 64e-6, 0.1, 1.0, 480, 520),   # multi-pass transforms (up to ~1300 rows)
     (2 ** 21, 64e-6, 6.0, 12.0, 32, 36),    # downsampling factors 2930-5860 (direct above 4000)
+    (2 ** 20, 64e-6, 0.1, 10.0, 240, 241),  # 555 downsampling steps (three packed launches)
 ]
 
 
@@ -49,17 +50,28 @@ def test_periodogram_parity(oracle, case):
 
 
 @pytest.mark.parametrize("case", [(2 ** 18, 128e-6, 0.2, 2.0, 240, 260),
-                                  (2 ** 21, 64e-6, 6.0, 12.0, 32, 36)])
+                                  (2 ** 21, 64e-6, 6.0, 12.0, 32, 36),
+                                  (2 ** 20, 64e-6, 0.1, 10.0, 240, 241)])
 def test_plain_schedule_matches(oracle, case):
+    """The pass-by-pass schedule and the batch of one give bit-identical
+    results (the same task code over the same downsampled series), which
+    equal the oracle's: periods and fold bins exactly, S/N within the
+    tolerance.  The third case has more downsampling steps than one packed
+    launch holds."""
     from riptide_amd import _lib, libcpp
     N, tsamp, pmin, pmax, bmin, bmax = case
     x = oracle.normalise(_pulsar(N, tsamp, 0.77, 0.02, 20.0, seed=5))
     w = oracle.generate_width_trials(bmin)
     d = _lib.to_device(x)
-    a = libcpp.periodogram_device(d, N, tsamp, w, pmin, pmax, bmin, bmax, plain=False)
-    b = libcpp.periodogram_device(d, N, tsamp, w, pmin, pmax, bmin, bmax, plain=True)
+    a = [_lib.to_numpy(u) for u in
+         libcpp.periodogram_device(d, N, tsamp, w, pmin, pmax, bmin, bmax, plain=False)]
+    b = [_lib.to_numpy(u) for u in
+         libcpp.periodogram_device(d, N, tsamp, w, pmin, pmax, bmin, bmax, plain=True)]
     for u, v in zip(a, b):
-        assert np.array_equal(_lib.to_numpy(u), _lib.to_numpy(v))
+        assert np.array_equal(u, v)
+    Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    assert np.array_equal(a[0], Po) and np.array_equal(a[1], Fo)
+    np.testing.assert_allclose(a[2], So, rtol=SNR_RTOL, atol=SNR_ATOL)
 
 
 def _ulps(a, b):

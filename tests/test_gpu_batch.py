@@ -118,6 +118,41 @@ def test_batch_many_downsampling_steps(oracle):
     b.close()
 
 
+def test_batch_wide_bins_classes(oracle):
+    """The reference's third search range (bins 960-1040, pipeline/config/
+    example.yaml:153-154) and a wider class (2000-2200 bins) in one batch,
+    with the untuned default tiling for those classes: every series equals
+    the oracle, eager and from the graph."""
+    import torch
+    from riptide_amd import _lib
+    from riptide_amd.batch import BatchSearch
+
+    rng = np.random.default_rng(960)
+    tsamp, pmin, pmax = 256e-6, 0.6, 5.0
+    sizes = [2 ** 19, 2 ** 19 - 3333, 2 ** 19 + 2048]
+    bmin = np.array([960, 2000, 960])
+    bmax = np.array([1040, 2200, 1040])
+    data = [_series(rng, n, tsamp) for n in sizes]
+    b = BatchSearch(sizes, tsamp, bmin, bmax, pmin, pmax, rmed_width=4.0, group_size=3)
+    xs = [_lib.to_device(x) for x in data]
+    outs = b.allocate_outputs()
+    b.bind(xs, outs)
+    want = [oracle.ffa_search(x, tsamp, period_min=pmin, period_max=pmax, bins_min=int(bmin[i]),
+                              bins_max=int(bmax[i]), rmed_width=4.0)
+            for i, x in enumerate(data)]
+    for use_graph in (False, True, True):
+        for o in outs:
+            o.fill_(np.nan)
+        b.run(use_graph=use_graph)
+        b.check()
+        torch.cuda.synchronize()
+        for i, (_, w, P, F, S) in enumerate(want):
+            Pb, Fb = b.grid(i)
+            assert np.array_equal(Pb, P) and np.array_equal(Fb, F)
+            np.testing.assert_allclose(_lib.to_numpy(outs[i]), S, rtol=1e-5, atol=1e-4)
+    b.close()
+
+
 def test_ffa_search_batch_api(oracle):
     import riptide_amd as rt
     from riptide_amd.batch import ffa_search_batch

@@ -105,6 +105,7 @@ static std::vector<int> cone_depths(int d1, int cap, int lmax, int ext_rows)
 
 std::vector<int> transform_pass_depths(int64_t m, int cap, int lmax, int ext_rows)
 {
+    if (cap < 1) return {};
     if (m <= cap) return {0};
     return cone_depths(leaf_depth(m, cap), cap, lmax, ext_rows);
 }
@@ -119,6 +120,7 @@ int64_t ffa_adds(int64_t m)
 int count_transform_passes(int64_t m, int bins, int cap, int lmax, int ext_rows)
 {
     (void)bins;
+    if (cap < 1) return -1;  // rows too wide for the shape's tile
     if (m <= cap) return 1;
     const int d1 = leaf_depth(m, cap);
     const std::vector<int> depths = cone_depths(d1, cap, lmax, ext_rows);
@@ -147,6 +149,7 @@ int plan_transform_tasks(const TransformSpec& s, std::vector<std::vector<FfaTask
     auto ensure = [&](size_t n) {
         if (passes.size() < n) passes.resize(n);
     };
+    if (s.cap < 1) return -1;  // rows too wide for the shape's tile
     if (s.m <= s.cap) {
         ensure(1);
         FfaTask t = base_task(s);

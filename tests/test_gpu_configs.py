@@ -119,7 +119,8 @@ def test_c4_gradient_full_size():
     L64 = _fp64_loss(x64, tsamp, w, pr, p0, frac)
     L64.backward()
     g_ref = x64.grad.numpy()
-    assert abs(float(loss) - float(L64.detach())) < 1e-4 * abs(float(L64.detach()))
+    loss_gpu, loss_ref = float(loss.detach()), float(L64.detach())
+    assert abs(loss_gpu - loss_ref) < 1e-4 * abs(loss_ref)
     rel = np.linalg.norm(g_gpu - g_ref) / np.linalg.norm(g_ref)
     assert rel < 1e-4, rel
     rng = np.random.default_rng(64)

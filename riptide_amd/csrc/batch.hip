@@ -70,7 +70,8 @@ struct TileClass {
 std::vector<TileClass> g_tile_classes;
 
 // The tiling of a transform: the first class that holds it, else the
-// current shape with the largest tile.
+// current shape with the largest tile; rows too wide for that shape's
+// staging take the shape with the deepest staging that holds them.
 void tile_for(int64_t rows, int64_t bins, int default_shape, int* shape, int* cap, int* lmax)
 {
     *shape = default_shape;
@@ -84,7 +85,19 @@ void tile_for(int64_t rows, int64_t bins, int default_shape, int* shape, int* ca
             break;
         }
     }
-    const int most = ffa_cap_rows((int)bins, *shape);
+    int most = ffa_cap_rows((int)bins, *shape);
+    if (most < 1) {
+        int best = -1;
+        for (int sh = 0; sh < kFfaShapes; ++sh)
+            if (ffa_cap_rows((int)bins, sh) > 0 &&
+                (best < 0 || kFfaShape[sh].stage > kFfaShape[best].stage))
+                best = sh;
+        if (best >= 0) {
+            *shape = best;
+            *cap = 0;
+            most = ffa_cap_rows((int)bins, best);
+        }
+    }
     *cap = (*cap > 0 && *cap < most) ? *cap : most;
 }
 

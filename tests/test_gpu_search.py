@@ -76,24 +76,30 @@ def test_plain_schedule_matches(oracle, case):
 
 def test_periodogram_rows_wider_than_the_tile(oracle):
     """Rows of 3000-3100 bins (47-49 columns per lane) do not fit the staging
-    of workgroup shapes 0 and 1 (40 columns per lane): the search raises the
-    library's error instead of tiling; shape 2 (72 columns per lane, rows up
-    to 4095 bins) runs it and equals the oracle."""
-    from riptide_amd import libcpp, tune
+    of workgroup shapes 0 and 1 (40 columns per lane): the batch of one
+    tiles them with shape 2 (72 columns per lane, rows up to 4095 bins) and
+    equals the oracle, also with shape 0 asked for; the pass-by-pass
+    schedule, which keeps the shape asked for, raises the library's error."""
+    from riptide_amd import _lib, libcpp, tune
     N, tsamp, pmin, pmax, bmin, bmax = 2 ** 18, 256e-6, 0.8, 3.0, 3000, 3100
     x = oracle.normalise(_pulsar(N, tsamp, 1.3, 0.03, 15.0, seed=3000))
     w = oracle.generate_width_trials(bmin)
+    Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+    for shape in (None, 0):
+        tune.set_shape(shape)
+        try:
+            P, F, S = libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+        finally:
+            tune.set_shape(None)
+        assert np.array_equal(P, Po) and np.array_equal(F, Fo)
+        np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)
     tune.set_shape(0)
     try:
         with pytest.raises(ValueError, match="bins too large for the FFA tiling"):
-            libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
-        tune.set_shape(2)
-        P, F, S = libcpp.periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
+            libcpp.periodogram_device(_lib.to_device(x), N, tsamp, w, pmin, pmax, bmin, bmax,
+                                      plain=True)
     finally:
         tune.set_shape(None)
-    Po, Fo, So = oracle.oracle().periodogram(x, tsamp, w, pmin, pmax, bmin, bmax)
-    assert np.array_equal(P, Po) and np.array_equal(F, Fo)
-    np.testing.assert_allclose(S, So, rtol=SNR_RTOL, atol=SNR_ATOL)
 
 
 def _ulps(a, b):

@@ -108,7 +108,9 @@ int rt_periodogram_plain(const float* d_in, int64_t size, double tsamp, const ui
  * scratch pool; no hipMalloc / hipFree.  Identical results.  Searches whose
  * intermediates exceed the array (roughly: packed downsampled series, the
  * largest transform's two intermediate states and 24 MiB of task list) are
- * rejected with RT_EINVAL.  Calls are serialised by the library's lock and
+ * rejected with RT_EINVAL, as are rows wider than the staging of the
+ * workgroup shape in effect (it runs pass by pass in that shape, where
+ * rt_periodogram tiles such rows with a deeper shape).  Calls are serialised by the library's lock and
  * wait for their work; the array keeps the last call's contents, and calls
  * that overlap on one GPU from several processes each use their own copy. */
 int rt_periodogram_noalloc(const float* d_in, int64_t size, double tsamp,
